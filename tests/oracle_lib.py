@@ -1,0 +1,180 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure only: the oracle is the checker for the HIP path and is
+never imported by the product package.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+
+NCONSTS = 15
+HIST_COLS = 6
+DECK_NOUT = 33
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        _lib = ctypes.CDLL(ORACLE_SO)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(L):
+    P = ctypes.c_void_p
+    Lg = ctypes.c_long
+    I = ctypes.c_int
+    for sfx, R in (("_f", ctypes.c_float), ("_d", ctypes.c_double)):
+        g = lambda n: getattr(L, n + sfx)
+        g("ora_load_interpolator").argtypes = [P, P, Lg, Lg, Lg]
+        g("ora_clear_accumulator").argtypes = [P, Lg]
+        g("ora_push").argtypes = [P] * 8 + [Lg, P, P] + [R] * 5 + [Lg, Lg, Lg, I]
+        g("ora_push").restype = Lg
+        g("ora_push_omp").argtypes = [P] * 8 + [Lg, P, P] + [R] * 5 + [Lg, Lg, Lg, I, I]
+        g("ora_push_omp").restype = Lg
+        g("ora_unload_accumulator").argtypes = [P, P, Lg, Lg, Lg, R, R, R, R]
+        g("ora_fold_ghosts").argtypes = [P, P, P, Lg, Lg, Lg]
+        g("ora_copy_ghosts").argtypes = [P, P, P, Lg, Lg, Lg]
+        g("ora_advance_b").argtypes = [P, R, R, R, Lg, Lg, Lg]
+        g("ora_advance_e").argtypes = [P, R, R, R, Lg, Lg, Lg, R]
+        g("ora_energies").argtypes = [P, Lg, Lg, Lg, P, P, P, P]
+        g("ora_kinetic_energy").argtypes = [P, P, P, P, Lg, R, R]
+        g("ora_kinetic_energy").restype = ctypes.c_double
+        g("ora_default_particle_init").argtypes = [P] * 8 + [Lg, Lg, Lg, R, Lg, R, R]
+        g("ora_run").argtypes = [P] * 10 + [P, Lg, Lg, Lg, Lg, P, Lg, I, P]
+        g("ora_run").restype = Lg
+        g("ora_deck_2stream_em").argtypes = [P]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def sfx(dtype):
+    return "_f" if np.dtype(dtype) == np.float32 else "_d"
+
+
+class OracleState:
+    """Host state of one simulation in the layouts the oracle expects."""
+
+    def __init__(self, nx, ny, nz, np_, dtype):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.dtype = np.dtype(dtype)
+        self.ng = (nx + 2) * (ny + 2) * (nz + 2)
+        self.fields = np.zeros((9, self.ng), self.dtype)
+        self.interp = np.zeros((18, self.ng), self.dtype)
+        self.acc = np.zeros((self.ng, 3, 4), self.dtype)
+        self.p = {k: np.zeros(np_, self.dtype) for k in ("x", "y", "z", "ux", "uy", "uz", "w")}
+        self.cell = np.zeros(np_, np.int32)
+        self.np = np_
+
+    def copy(self):
+        o = OracleState.__new__(OracleState)
+        o.__dict__.update(self.__dict__)
+        o.fields = self.fields.copy()
+        o.interp = self.interp.copy()
+        o.acc = self.acc.copy()
+        o.p = {k: v.copy() for k, v in self.p.items()}
+        o.cell = self.cell.copy()
+        return o
+
+    def _pp(self):
+        return [_p(self.p[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")] + [_p(self.cell)]
+
+    def fn(self, name):
+        return getattr(lib(), name + sfx(self.dtype))
+
+    def load_interpolator(self):
+        self.fn("ora_load_interpolator")(_p(self.fields), _p(self.interp), self.nx, self.ny, self.nz)
+
+    def clear_accumulator(self):
+        self.fn("ora_clear_accumulator")(_p(self.acc), self.ng)
+
+    def push(self, qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, boundary=1, nthreads=1):
+        if nthreads > 1:
+            return self.fn("ora_push_omp")(*self._pp(), self.np, _p(self.interp), _p(self.acc),
+                                           qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp,
+                                           self.nx, self.ny, self.nz, boundary, nthreads)
+        return self.fn("ora_push")(*self._pp(), self.np, _p(self.interp), _p(self.acc),
+                                   qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp,
+                                   self.nx, self.ny, self.nz, boundary)
+
+    def unload(self, dx, dy, dz, dt):
+        self.fn("ora_unload_accumulator")(_p(self.fields), _p(self.acc), self.nx, self.ny, self.nz,
+                                          dx, dy, dz, dt)
+
+    def advance_b(self, px, py, pz):
+        self.fn("ora_advance_b")(_p(self.fields), px, py, pz, self.nx, self.ny, self.nz)
+
+    def advance_e(self, px, py, pz, dt_eps0):
+        self.fn("ora_advance_e")(_p(self.fields), px, py, pz, self.nx, self.ny, self.nz, dt_eps0)
+
+    def fold_ghosts(self):
+        f = self.fields
+        self.fn("ora_fold_ghosts")(_p(f[6]), _p(f[7]), _p(f[8]), self.nx, self.ny, self.nz)
+
+    def copy_ghosts(self, first):
+        f = self.fields
+        self.fn("ora_copy_ghosts")(_p(f[first]), _p(f[first + 1]), _p(f[first + 2]),
+                                   self.nx, self.ny, self.nz)
+
+    def energies(self):
+        e64 = ctypes.c_double()
+        b64 = ctypes.c_double()
+        R = ctypes.c_float if self.dtype == np.float32 else ctypes.c_double
+        er, br = R(), R()
+        self.fn("ora_energies")(_p(self.fields), self.nx, self.ny, self.nz,
+                                ctypes.byref(e64), ctypes.byref(b64), ctypes.byref(er), ctypes.byref(br))
+        return e64.value, b64.value, er.value, br.value
+
+    def kinetic_energy(self, me=1.0, c=1.0):
+        return self.fn("ora_kinetic_energy")(_p(self.p["ux"]), _p(self.p["uy"]), _p(self.p["uz"]),
+                                             _p(self.p["w"]), self.np, me, c)
+
+    def run(self, consts, nsteps, nthreads=1, hist=True):
+        k = np.asarray(consts, self.dtype)
+        assert k.size == NCONSTS
+        h = np.zeros((nsteps, HIST_COLS)) if hist else None
+        moved = self.fn("ora_run")(_p(self.fields), _p(self.interp), _p(self.acc), *self._pp(),
+                                   self.np, self.nx, self.ny, self.nz, _p(k), nsteps, nthreads,
+                                   _p(h) if hist else None)
+        return h, moved
+
+
+def deck_2stream_em(dtype):
+    """Constants of the 2stream-em regression deck, computed by the oracle."""
+    out = np.zeros(DECK_NOUT)
+    getattr(lib(), "ora_deck_2stream_em" + sfx(dtype))(_p(out))
+    names = ["nx", "ny", "nz", "nppc", "num_steps", "np", "len_x", "len_y", "len_z",
+             "dx", "dy", "dz", "dt", "n0", "v0", "Npe", "we", "dxp"]
+    d = {n: out[i] for i, n in enumerate(names)}
+    for n in ("nx", "ny", "nz", "nppc", "num_steps", "np"):
+        d[n] = int(d[n])
+    d["consts"] = out[len(names):]
+    return d
+
+
+def default_particle_init(st, dxp, nppc, we, v0):
+    st.fn("ora_default_particle_init")(*st._pp(), st.np, st.nx, st.ny, dxp, nppc, we, v0)
+
+
+def setup_2stream_em(dtype):
+    """Initial state of the 2stream-em deck: zero fields, default particles."""
+    d = deck_2stream_em(dtype)
+    st = OracleState(d["nx"], d["ny"], d["nz"], d["np"], dtype)
+    default_particle_init(st, d["dxp"], d["nppc"], d["we"], d["v0"])
+    return d, st
