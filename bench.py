@@ -1,0 +1,180 @@
+#!/usr/bin/env python3
+"""Throughput of the PIC step on one MI355X: particle-steps/s.
+
+Workload: the 128^3 x 32 ppc Weibel-type configuration (67,108,864
+electrons, bi-Maxwellian u_th = (0.3, 0.05, 0.05) c, seed 11, dx = 0.1,
+dt = 0.99 Courant), FP32 push/deposit.  A step is one pass of the reference
+loop body (example/example.cpp:216-279): interpolator load, accumulator
+clear, fused push + move_p + deposit, unload, advance_b/2, advance_e,
+advance_b/2, FP64 field-energy reduction.  Inputs are resident in HBM before
+the timed region.  The FP64 path is timed on the same workload for the
+FP32/FP64 ratio, and the CPU baseline is the oracle (OpenMP) on a bounded
+number of steps of the same workload.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, AMD published
+
+
+def algorithmic_bytes(np_, nc, itemsize):
+    """Per-step algorithmic HBM bytes: particles read x,y,z,ux,uy,uz,w,cell
+    and write x,y,z,ux,uy,uz; per interior cell E,B read+write, J write+read,
+    energy read of E,B."""
+    per_p = 7 * itemsize + 4 + 6 * itemsize
+    per_c = 24 * itemsize
+    return per_p * np_, per_c * nc
+
+
+def run_gpu(deck, steps, warmup, sort_interval, history):
+    from cabanapic_amd import Simulation
+    parts = deck.load_particles()
+    sim = Simulation(deck.nx, deck.ny, deck.nz, deck.num_particles, deck.R)
+    sim.upload_particles(*[parts[k] for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
+                         parts["cell"])
+    del parts
+    sp = deck.step_params(energy_interval=1, sort_interval=sort_interval)
+    if warmup:
+        sim.step(sp, warmup)
+    sim.set_timing(True)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    hist = sim.step(sp, steps, history=history)
+    sim.synchronize()
+    t1 = time.perf_counter()
+    stages = sim.stage_times()
+    errs = sim.device_errors()
+    sim.close()
+    return t1 - t0, hist, stages, errs
+
+
+def cpu_baseline(deck, nsteps, nthreads):
+    """The oracle's OpenMP step on the full workload for nsteps steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    parts = deck.load_particles()
+    st = oracle_lib.OracleState(deck.nx, deck.ny, deck.nz, deck.num_particles, deck.R)
+    for k in ("x", "y", "z", "ux", "uy", "uz", "w"):
+        st.p[k] = parts[k]
+    st.cell = parts["cell"]
+    del parts
+    k = deck.step_params()
+    consts = [k[n] for n in ("qdt_2mc", "cdt_dx", "cdt_dy", "cdt_dz", "qsp", "dx", "dy", "dz",
+                             "dt", "px", "py", "pz", "dt_eps0", "me", "c")]
+    st.run(consts, 1, nthreads=nthreads)  # first touch of the arrays
+    t0 = time.perf_counter()
+    st.run(consts, nsteps, nthreads=nthreads)
+    dt = time.perf_counter() - t0
+    return deck.num_particles * nsteps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=128, help="cells per axis")
+    ap.add_argument("--nppc", type=int, default=32)
+    ap.add_argument("--sort-interval", type=int, default=0)
+    ap.add_argument("--fp64-steps", type=int, default=20,
+                    help="steps of the FP64 path for the FP32/FP64 ratio (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=2,
+                    help="steps of the CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
+              "multi-GPU path; see DESIGN.md for how the step would shard.", file=sys.stderr)
+        sys.exit(2)
+
+    from cabanapic_amd import decks
+
+    d32 = decks.weibel_3d(np.float32, n=args.n, nppc=args.nppc)
+    np_, nc = d32.num_particles, d32.nx * d32.ny * d32.nz
+    print(f"# workload {d32.name}: {np_} particles, {nc} cells, dt={float(d32.dt):.6g}",
+          file=sys.stderr, flush=True)
+
+    t32, h32, st32, err32 = run_gpu(d32, args.steps, args.warmup, args.sort_interval, True)
+    if err32:
+        raise SystemExit(f"device error flags {err32}")
+    value = np_ * args.steps / t32
+    ms_step = 1e3 * t32 / args.steps
+    print(f"# fp32: {ms_step:.3f} ms/step, {value:.4g} particle-steps/s", file=sys.stderr,
+          flush=True)
+
+    bp, bc = algorithmic_bytes(np_, nc, 4)
+    push_n, push_ms = st32["push"]
+    push_avg_s = push_ms / max(push_n, 1) * 1e-3
+    push_gbs = bp / push_avg_s / 1e9
+    step_gbs = (bp + bc) / (t32 / args.steps) / 1e9
+
+    extra = {"stage_ms_per_step": {k: round(v[1] / max(v[0], 1), 4) for k, v in st32.items()
+                                   if v[0]},
+             "step_hbm_gbs": round(step_gbs, 1),
+             "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4)}
+
+    if args.fp64_steps > 0:
+        d64 = decks.weibel_3d(np.float64, n=args.n, nppc=args.nppc)
+        nwin = min(args.fp64_steps, args.steps)
+        t64, h64, st64, err64 = run_gpu(d64, args.fp64_steps, args.warmup, args.sort_interval,
+                                        True)
+        v64 = np_ * args.fp64_steps / t64
+        print(f"# fp64: {1e3 * t64 / args.fp64_steps:.3f} ms/step, {v64:.4g} particle-steps/s",
+              file=sys.stderr, flush=True)
+        # field energy of the FP32 run against the FP64 run of the same deck,
+        # over the steps both timed (same warmup, so the same step numbers)
+        e32 = h32[:nwin, 0] + h32[:nwin, 1]
+        e64 = h64[:nwin, 0] + h64[:nwin, 1]
+        rel = np.abs(e32 - e64) / np.abs(e64)
+        extra.update(fp64_value=v64, fp32_over_fp64=round(value / v64, 3),
+                     field_energy_rel_err_vs_fp64=float(rel.max()),
+                     field_energy_window=f"steps {args.warmup + 1}-{args.warmup + nwin}")
+
+    cpu = None
+    if args.cpu_steps > 0:
+        nth = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        v_cpu, t_cpu = cpu_baseline(d32, args.cpu_steps, nth)
+        cpu = {"value": v_cpu, "unit": "particle-steps/s", "cores": nth, "kind": "port",
+               "sample": f"oracle OpenMP step (C restatement, FP32) on the full "
+                         f"{d32.name} workload, {args.cpu_steps} steps, {t_cpu:.1f} s"}
+        print(f"# cpu: {v_cpu:.4g} particle-steps/s on {nth} threads", file=sys.stderr,
+              flush=True)
+
+    line = {
+        "metric": "particle-steps/s at 128^3x32ppc, FP32 vs plain FP64 (<=1e-3 rel energy err)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": d32.name, "cells": [d32.nx, d32.ny, d32.nz], "nppc": d32.nppc,
+                   "particles": np_, "sort_interval": args.sort_interval,
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(push_gbs, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(push_gbs / HBM_PEAK_GBS, 4),
+                     "kernel": "k_push", "traffic": None},
+        "cpu_baseline": cpu,
+        "extra": extra,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,179 @@
+"""Input decks of the benchmark and test configurations.
+
+``Deck`` mirrors the parameter block of the reference's ``_Input_Deck``
+(src/input/deck.h:161-348): the same member names and defaults, the same
+``courant_length`` and ``derive_params``, and ``driver_constants`` evaluates
+the constants of example/example.cpp:77-114,179-181 with the reference's
+real_t arithmetic (numpy scalars of the deck's dtype; double literals are
+applied in float64 exactly where the C++ source promotes).
+
+Particle loading for the synthetic configurations goes through the
+library's counter-based plasma loader (``cpic_init_plasma``), so the C++
+driver and Python produce bit-identical initial states.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _capi
+
+F64 = np.float64
+
+
+class Deck:
+    def __init__(self, dtype=np.float32):
+        R = self.R = np.dtype(dtype).type
+        # _Input_Deck member defaults (deck.h:254-309)
+        self.de = R(1.0)
+        self.ec = R(1.0)
+        self.me = R(1.0)
+        self.mu = R(1.0)
+        self.c = R(1.0)
+        self.eps = R(1.0)
+        self.qsp = R(-self.ec)
+        self.n0 = R(1.0)
+        self.nx, self.ny, self.nz = 16, 1, 1
+        self.num_ghosts = 1
+        self.nppc = 1
+        self.dt = R(1.0)
+        self.num_steps = 2
+        self.len_x_global = R(1.0)
+        self.len_y_global = R(1.0)
+        self.len_z_global = R(1.0)
+        self.Npe = R(-1)
+        self.Ne = R(-1)
+        self.v0 = R(1.0)
+        self.num_particles = -1
+        self.plasma = None  # cpic_plasma_spec fields for synthetic decks
+        self.name = "default"
+
+    # deck.h:191-198
+    def courant_length(self, lx, ly, lz, nx, ny, nz):
+        R = self.R
+        w1 = R(0)
+        for n, l in ((nx, lx), (ny, ly), (nz, lz)):
+            if n > 1:
+                w0 = R(R(n) / R(l))
+                w1 = R(w1 + R(w0 * w0))
+        # sqrt of a real_t: double rounding of a correctly rounded sqrt is
+        # innocuous, so float64 sqrt then rounding equals the float sqrt
+        return R(math.sqrt(F64(R(R(1) / w1))))
+
+    # deck.h:323-348
+    def derive_params(self):
+        R = self.R
+        self.len_x, self.len_y, self.len_z = self.len_x_global, self.len_y_global, self.len_z_global
+        self.dx = R(self.len_x / R(self.nx))
+        self.dy = R(self.len_y / R(self.ny))
+        self.dz = R(self.len_z / R(self.nz))
+        g = 2 * self.num_ghosts
+        self.num_cells = (self.nx + g) * (self.ny + g) * (self.nz + g)
+        if self.num_particles < 0:
+            self.num_particles = self.nx * self.ny * self.nz * self.nppc
+            if self.Ne < 0:
+                self.Ne = R(self.num_particles)
+        if self.Npe < 0:
+            self.Npe = R(R(R(self.n0 * self.len_x_global) * self.len_y_global) * self.len_z_global)
+        return self
+
+    # example/example.cpp:77-114,179-181
+    def driver_constants(self):
+        R = self.R
+        c, dt = self.c, self.dt
+        k = {}
+        k["dxp"] = R(np.float32(2.0) / np.float32(self.nppc))
+        k["qdt_2mc"] = R(R(self.qsp * dt) / R(R(R(2) * self.me) * c))
+        k["cdt_dx"] = R(R(c * dt) / self.dx)
+        k["cdt_dy"] = R(R(c * dt) / self.dy)
+        k["cdt_dz"] = R(R(c * dt) / self.dz)
+        k["dt_eps0"] = R(dt / self.eps)
+        frac = R(1.0)
+        k["we"] = R(R(self.Npe) / R(int(self.Ne)))
+        k["px"] = R(R(R(frac * c) * dt) / self.dx) if self.nx > 1 else R(0)
+        k["py"] = R(R(R(frac * c) * dt) / self.dy) if self.ny > 1 else R(0)
+        k["pz"] = R(R(R(frac * c) * dt) / self.dz) if self.nz > 1 else R(0)
+        k.update(qsp=self.qsp, dx=self.dx, dy=self.dy, dz=self.dz, dt=dt, me=self.me, c=c)
+        return k
+
+    def step_params(self, **kw):
+        k = self.driver_constants()
+        names = ["qdt_2mc", "cdt_dx", "cdt_dy", "cdt_dz", "qsp", "dx", "dy", "dz", "dt",
+                 "px", "py", "pz", "dt_eps0", "me", "c"]
+        d = {n: float(k[n]) for n in names}
+        d.update(kw)
+        return d
+
+    def load_particles(self):
+        """Initial particles of a synthetic deck (SoA dict + int32 cell)."""
+        if self.plasma is None:
+            raise ValueError(f"deck {self.name} has no synthetic plasma")
+        n = self.num_particles
+        out = {k: np.empty(n, self.R) for k in ("x", "y", "z", "ux", "uy", "uz", "w")}
+        out["cell"] = np.empty(n, np.int32)
+        spec = _capi.PlasmaSpec()
+        p = self.plasma
+        spec.seed = p["seed"]
+        spec.nx, spec.ny, spec.nz, spec.nppc = self.nx, self.ny, self.nz, self.nppc
+        spec.weight = float(self.driver_constants()["we"])
+        for i in range(3):
+            spec.uth[i] = p.get("uth", (0, 0, 0))[i]
+            spec.drift[i] = p.get("drift", (0, 0, 0))[i]
+        spec.pert_amp = p.get("pert_amp", 0.0)
+        spec.len_x = float(self.len_x)
+        spec.pert_mode = p.get("pert_mode", 0)
+        spec.counter_streaming = 1 if p.get("counter_streaming") else 0
+        ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+        _capi.check(_capi.load().cpic_init_plasma(
+            ctypes.byref(spec), np.dtype(self.R).itemsize, n,
+            *[ptr(out[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")], ptr(out["cell"])))
+        return out
+
+
+def _cube(deck, n, nppc, dx, steps):
+    R = deck.R
+    deck.nx = deck.ny = deck.nz = n
+    deck.nppc = nppc
+    deck.num_steps = steps
+    L = R(F64(n) * F64(dx))
+    deck.len_x_global = deck.len_y_global = deck.len_z_global = L
+    deck.dt = R(F64(0.99) * F64(deck.courant_length(L, L, L, n, n, n)) / F64(deck.c))
+    return deck
+
+
+def weibel_3d(dtype=np.float32, n=128, nppc=32, steps=500, seed=11):
+    """Precision/metric configuration: 128^3 periodic cells, 32 ppc,
+    bi-Maxwellian electrons u_th = (0.3, 0.05, 0.05) c, immobile ions,
+    dx = 0.1 c/w_pe, dt = 0.99 Courant."""
+    d = _cube(Deck(dtype), n, nppc, 0.1, steps)
+    d.name = f"weibel_{n}^3x{nppc}"
+    d.n0 = d.R(1.0)
+    d.plasma = dict(seed=seed, uth=(0.3, 0.05, 0.05))
+    return d.derive_params()
+
+
+def two_stream_3d(dtype=np.float32, n=64, nppc=64, steps=200, seed=7, v0=0.2, uth=0.01,
+                  amp=1e-3):
+    """3D two-stream: counter-streaming beams +-v0 along x (nppc/2 each),
+    Maxwellian spread uth per component, mode-1 drift perturbation."""
+    d = Deck(dtype)
+    R = d.R
+    d.nx = d.ny = d.nz = n
+    d.nppc = nppc
+    d.num_steps = steps
+    d.v0 = R(v0)
+    L = R(2.0 * math.pi * v0 / 0.5)  # fastest-growing two-stream wavelength scale
+    d.len_x_global = d.len_y_global = d.len_z_global = L
+    d.dt = R(F64(0.99) * F64(d.courant_length(L, L, L, n, n, n)) / F64(d.c))
+    d.n0 = R(1.0)
+    u0 = v0 / math.sqrt(1.0 - v0 * v0)
+    d.name = f"two_stream_{n}^3x{nppc}"
+    d.plasma = dict(seed=seed, uth=(uth, uth, uth), drift=(u0, 0.0, 0.0), pert_amp=amp,
+                    pert_mode=1, counter_streaming=True)
+    return d.derive_params()
+
+
+CONFIGS = {
+    "weibel128": weibel_3d,
+    "twostream64": two_stream_3d,
+}

@@ -1,0 +1,228 @@
+"""Host-side mirror of the reference's hot-path interface.
+
+``Simulation`` owns one device context (particles, fields, interpolators and
+accumulators resident in HBM) and exposes the reference's stage functions
+under their reference names, with the same argument meaning:
+
+=============================  ==============================================
+reference                      here
+=============================  ==============================================
+load_interpolator_array        Simulation.load_interpolator_array()
+  (src/interpolator.h:14-21)
+clear_accumulator_array        Simulation.clear_accumulator_array()
+  (src/accumulator.h:15-21)
+push (src/push.h:7-23)         Simulation.push(qdt_2mc, cdt_dx, cdt_dy,
+                               cdt_dz, qsp, boundary)
+unload_accumulator_array       Simulation.unload_accumulator_array(dx, dy, dz,
+  (src/accumulator.h:23-34)    dt)
+Field_Solver::advance_b        Simulation.advance_b(px, py, pz)
+  (src/fields.h:352-364)
+Field_Solver::advance_e        Simulation.advance_e(px, py, pz, dt_eps0)
+  (src/fields.h:365-378)
+e_energy / b_energy            Simulation.e_energy(), .b_energy()
+  (src/fields.h:556-615)
+main loop body                 Simulation.step(params, nsteps)
+  (example/example.cpp:216-279)
+=============================  ==============================================
+
+Errors raise ``CpicError``; the reference's ``exit(1)`` on a Reflect boundary
+(src/fields.h:21-25) becomes CPIC_ERR_UNSUPPORTED.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _capi
+from ._capi import BOUNDARY_PERIODIC, CpicError, StepParams, check
+
+PARTICLE_MEMBERS = ("x", "y", "z", "ux", "uy", "uz", "w")
+FIELD_MEMBERS = ("ex", "ey", "ez", "cbx", "cby", "cbz", "jfx", "jfy", "jfz")
+INTERP_MEMBERS = ("ex", "dexdy", "dexdz", "d2exdydz", "ey", "deydz", "deydx", "d2eydzdx",
+                  "ez", "dezdx", "dezdy", "d2ezdxdy", "cbx", "dcbxdx", "cby", "dcbydy",
+                  "cbz", "dcbzdz")
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Simulation:
+    """One simulation's device state on the current HIP device."""
+
+    def __init__(self, nx, ny, nz, max_particles, dtype=np.float32, num_ghosts=1):
+        self.lib = _capi.load()
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float32, np.float64):
+            raise ValueError("dtype must be float32 or float64")
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.ncells = (self.nx + 2) * (self.ny + 2) * (self.nz + 2)
+        cfg = _capi.Config(self.dtype.itemsize, num_ghosts, self.nx, self.ny, self.nz,
+                           int(max_particles))
+        h = ctypes.c_void_p()
+        check(self.lib.cpic_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+
+    # -- lifetime -------------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.cpic_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def synchronize(self):
+        check(self.lib.cpic_synchronize(self._h))
+
+    # -- data movement ----------------------------------------------------------
+    def _arr(self, a, dtype=None):
+        return np.ascontiguousarray(a, dtype=self.dtype if dtype is None else dtype)
+
+    def upload_particles(self, x, y, z, ux, uy, uz, w, cell):
+        arrs = [self._arr(a) for a in (x, y, z, ux, uy, uz, w)]
+        c = self._arr(cell, np.int32)
+        n = c.size
+        if any(a.size != n for a in arrs):
+            raise ValueError("particle arrays differ in length")
+        check(self.lib.cpic_upload_particles(self._h, n, *[_ptr(a) for a in arrs], _ptr(c)))
+
+    def num_particles(self):
+        n = ctypes.c_int64()
+        check(self.lib.cpic_num_particles(self._h, ctypes.byref(n)))
+        return n.value
+
+    def download_particles(self):
+        n = self.num_particles()
+        out = {k: np.empty(n, self.dtype) for k in PARTICLE_MEMBERS}
+        out["cell"] = np.empty(n, np.int32)
+        check(self.lib.cpic_download_particles(
+            self._h, n, *[_ptr(out[k]) for k in PARTICLE_MEMBERS], _ptr(out["cell"])))
+        return out
+
+    def upload_fields(self, fields):
+        f = self._arr(fields)
+        if f.size != 9 * self.ncells:
+            raise ValueError("fields must hold 9 x Ng values")
+        check(self.lib.cpic_upload_fields(self._h, _ptr(f)))
+
+    def download_fields(self):
+        f = np.empty((9, self.ncells), self.dtype)
+        check(self.lib.cpic_download_fields(self._h, _ptr(f)))
+        return f
+
+    def upload_interpolators(self, interp):
+        a = self._arr(interp)
+        if a.size != 18 * self.ncells:
+            raise ValueError("interpolators must hold 18 x Ng values")
+        check(self.lib.cpic_upload_interpolators(self._h, _ptr(a)))
+
+    def download_interpolators(self):
+        a = np.empty((18, self.ncells), self.dtype)
+        check(self.lib.cpic_download_interpolators(self._h, _ptr(a)))
+        return a
+
+    def upload_accumulators(self, acc):
+        a = self._arr(acc)
+        if a.size != 12 * self.ncells:
+            raise ValueError("accumulators must hold Ng x 3 x 4 values")
+        check(self.lib.cpic_upload_accumulators(self._h, _ptr(a)))
+
+    def download_accumulators(self):
+        a = np.empty((self.ncells, 3, 4), self.dtype)
+        check(self.lib.cpic_download_accumulators(self._h, _ptr(a)))
+        return a
+
+    # -- hot-path stages --------------------------------------------------------
+    def load_interpolator_array(self):
+        check(self.lib.cpic_load_interpolator_array(self._h))
+
+    def clear_accumulator_array(self):
+        check(self.lib.cpic_clear_accumulator_array(self._h))
+
+    def push(self, qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, boundary=BOUNDARY_PERIODIC):
+        check(self.lib.cpic_push(self._h, float(qdt_2mc), float(cdt_dx), float(cdt_dy),
+                                 float(cdt_dz), float(qsp), int(boundary)))
+
+    def unload_accumulator_array(self, dx, dy, dz, dt):
+        check(self.lib.cpic_unload_accumulator_array(self._h, float(dx), float(dy), float(dz),
+                                                     float(dt)))
+
+    def advance_b(self, px, py, pz):
+        check(self.lib.cpic_advance_b(self._h, float(px), float(py), float(pz)))
+
+    def advance_e(self, px, py, pz, dt_eps0):
+        check(self.lib.cpic_advance_e(self._h, float(px), float(py), float(pz), float(dt_eps0)))
+
+    def update_ghosts_j(self):
+        check(self.lib.cpic_update_ghosts_j(self._h))
+
+    def energies(self):
+        e, b = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.cpic_energies(self._h, ctypes.byref(e), ctypes.byref(b)))
+        return e.value, b.value
+
+    def e_energy(self):
+        return self.energies()[0]
+
+    def b_energy(self):
+        return self.energies()[1]
+
+    def kinetic_energy(self, me=1.0, c=1.0):
+        ke = ctypes.c_double()
+        check(self.lib.cpic_kinetic_energy(self._h, float(me), float(c), ctypes.byref(ke)))
+        return ke.value
+
+    def sort_particles(self):
+        check(self.lib.cpic_sort_particles(self._h))
+
+    def step(self, params, nsteps, history=True):
+        """Runs nsteps of the reference loop body; returns the (nsteps, 3)
+        energy history [E, B, KE] (NaN where no dump was taken)."""
+        if isinstance(params, dict):
+            params = make_step_params(**params)
+        hist = np.full((int(nsteps), 3), np.nan) if history else None
+        check(self.lib.cpic_step(self._h, ctypes.byref(params), int(nsteps),
+                                 _ptr(hist) if history else None))
+        return hist
+
+    # -- diagnostics ------------------------------------------------------------
+    def set_timing(self, enabled):
+        check(self.lib.cpic_set_timing(self._h, 1 if enabled else 0))
+
+    def stage_times(self):
+        counts = np.zeros(_capi.NUM_STAGES, np.int64)
+        ms = np.zeros(_capi.NUM_STAGES, np.float64)
+        check(self.lib.cpic_stage_times(self._h, _ptr(counts), _ptr(ms)))
+        return {name: (int(counts[i]), float(ms[i])) for i, name in enumerate(_capi.STAGES)}
+
+    def last_push_movers(self):
+        n = ctypes.c_int64()
+        check(self.lib.cpic_last_push_movers(self._h, ctypes.byref(n)))
+        return n.value
+
+    def device_errors(self):
+        f = ctypes.c_int32()
+        check(self.lib.cpic_device_errors(self._h, ctypes.byref(f)))
+        return f.value
+
+
+def make_step_params(qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, dx, dy, dz, dt, px, py, pz,
+                     dt_eps0, me=1.0, c=1.0, boundary=BOUNDARY_PERIODIC, energy_interval=1,
+                     sort_interval=0, kinetic_interval=0):
+    return StepParams(float(qdt_2mc), float(cdt_dx), float(cdt_dy), float(cdt_dz), float(qsp),
+                      float(dx), float(dy), float(dz), float(dt), float(px), float(py),
+                      float(pz), float(dt_eps0), float(me), float(c), int(boundary),
+                      int(energy_interval), int(sort_interval), int(kinetic_interval))
+
+
+__all__ = ["Simulation", "make_step_params", "CpicError", "PARTICLE_MEMBERS", "FIELD_MEMBERS",
+           "INTERP_MEMBERS"]

@@ -1,0 +1,216 @@
+/*
+ * cabanapic_amd.h -- C-ABI of the MI355X-native CabanaPIC hot path.
+ *
+ * One context owns the device state of one simulation (particles, fields,
+ * interpolators, accumulators) on the current HIP device and one HIP stream.
+ * Entry points are asynchronous on that stream unless they return host data
+ * (downloads, energies) or say otherwise.  All functions return CPIC_OK or an
+ * error code; cpic_last_error() describes the last failure of the calling
+ * thread.  No exception crosses the ABI.
+ *
+ * Real-valued arguments are passed as double; a float context converts them
+ * to float once (exact for values that were computed in float).  Host buffers
+ * passed to uploads/downloads hold real_t = float or double according to the
+ * context's real_size.
+ *
+ * Host buffer layouts (identical to the reference's logical layouts):
+ *   Ng = (nx+2)(ny+2)(nz+2), voxel = x + (nx+2)*(y + (ny+2)*z)
+ *        (src/types.h:184-195); ghost width 1.
+ *   fields        9 x Ng, component-major, order ex,ey,ez,cbx,cby,cbz,jfx,
+ *                 jfy,jfz (src/types.h:138-170)
+ *   interpolators 18 x Ng, component-major, order of src/types.h:64-84
+ *   accumulators  Ng x 3 x 4 (jx[4],jy[4],jz[4] per cell,
+ *                 src/types.h:109-120)
+ *   particles     SoA x,y,z,ux,uy,uz,w (real_t) + int32 cell
+ *                 (src/types.h:31-54); x,y,z are in-cell offsets in [-1,1].
+ */
+#ifndef CABANAPIC_AMD_H
+#define CABANAPIC_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CPIC_OK 0
+#define CPIC_ERR_INVALID 1     /* bad argument or input state            */
+#define CPIC_ERR_UNSUPPORTED 2 /* e.g. Reflect boundary (fields.h:21-25)  */
+#define CPIC_ERR_HIP 3         /* a HIP runtime call failed               */
+#define CPIC_ERR_DEVICE 4      /* a kernel flagged an invalid particle    */
+
+/* src/input/deck.h:9-12 */
+#define CPIC_BOUNDARY_REFLECT 0
+#define CPIC_BOUNDARY_PERIODIC 1
+
+typedef struct cpic_context cpic_context;
+
+typedef struct cpic_config {
+  int32_t real_size;     /* 4 = float, 8 = double (REAL_TYPE, types.h:4-8) */
+  int32_t num_ghosts;    /* must be 1 (deck.h:267)                         */
+  int64_t nx, ny, nz;    /* interior cells per axis                        */
+  int64_t max_particles; /* particle capacity of the context               */
+} cpic_config;
+
+/* Constants of one step, as computed by the driver
+ * (example/example.cpp:77-114,179-181). */
+typedef struct cpic_step_params {
+  double qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp; /* push arguments            */
+  double dx, dy, dz, dt;                       /* unload arguments          */
+  double px, py, pz, dt_eps0;                  /* field solver arguments    */
+  double me, c;                                /* kinetic energy units      */
+  int32_t boundary;         /* CPIC_BOUNDARY_*                              */
+  int32_t energy_interval;  /* E,B energies every k steps (0 = never);
+                               the reference dumps every step
+                               (ENERGY_DUMP_INTERVAL, example.cpp:53-55)   */
+  int32_t sort_interval;    /* cell sort every k steps (0 = never, as in the
+                               reference, example/example.cpp:226-228)      */
+  int32_t kinetic_interval; /* kinetic energy every k steps (0 = never)     */
+} cpic_step_params;
+
+/* ---- context ------------------------------------------------------------ */
+
+/* Replaces the allocations of example/example.cpp:121-144 and the zeroing
+ * of Field_Solver's constructor (src/fields.h:279-315) and of
+ * initialize_interpolator (src/interpolator.cpp:125-172): every device array
+ * starts at zero. */
+int cpic_create(const cpic_config *cfg, cpic_context **out);
+int cpic_destroy(cpic_context *ctx);
+const char *cpic_last_error(void);
+int cpic_synchronize(cpic_context *ctx);
+/* hipStream_t of the context, for callers that order their own work. */
+int cpic_get_stream(cpic_context *ctx, void **stream);
+
+/* ---- data movement (host <-> HBM), synchronous -------------------------- */
+
+/* Particle cells must be interior voxels; anything else is rejected. */
+int cpic_upload_particles(cpic_context *ctx, int64_t n, const void *x,
+                          const void *y, const void *z, const void *ux,
+                          const void *uy, const void *uz, const void *w,
+                          const int32_t *cell);
+int cpic_download_particles(cpic_context *ctx, int64_t n, void *x, void *y,
+                            void *z, void *ux, void *uy, void *uz, void *w,
+                            int32_t *cell);
+int cpic_num_particles(cpic_context *ctx, int64_t *n);
+int cpic_upload_fields(cpic_context *ctx, const void *fields);
+int cpic_download_fields(cpic_context *ctx, void *fields);
+int cpic_upload_interpolators(cpic_context *ctx, const void *interp);
+int cpic_download_interpolators(cpic_context *ctx, void *interp);
+int cpic_upload_accumulators(cpic_context *ctx, const void *acc);
+int cpic_download_accumulators(cpic_context *ctx, void *acc);
+
+/* ---- the hot-path stages (asynchronous) --------------------------------- */
+
+/* load_interpolator_array, src/interpolator.h:14-21,
+ * src/interpolator.cpp:4-124 */
+int cpic_load_interpolator_array(cpic_context *ctx);
+
+/* clear_accumulator_array, src/accumulator.h:15-21,
+ * src/accumulator.cpp:5-43 */
+int cpic_clear_accumulator_array(cpic_context *ctx);
+
+/* push<ScatterView> with the inline move_p, src/push.h:7-300,
+ * src/move_p.h:59-374: Boris push, cell crossing and charge-conserving
+ * deposit into the accumulators in one kernel.  boundary must be
+ * CPIC_BOUNDARY_PERIODIC. */
+int cpic_push(cpic_context *ctx, double qdt_2mc, double cdt_dx,
+              double cdt_dy, double cdt_dz, double qsp, int32_t boundary);
+
+/* unload_accumulator_array, src/accumulator.h:23-34,
+ * src/accumulator.cpp:45-122 */
+int cpic_unload_accumulator_array(cpic_context *ctx, double dx, double dy,
+                                  double dz, double dt);
+
+/* Field_Solver<EM_Field_Solver>::advance_b, src/fields.h:352-364,668-719
+ * (includes the periodic B ghost copy, serial_update_ghosts_B). */
+int cpic_advance_b(cpic_context *ctx, double px, double py, double pz);
+
+/* Field_Solver<EM_Field_Solver>::advance_e, src/fields.h:365-378,618-665
+ * (includes the periodic J fold, serial_update_ghosts, and J copy). */
+int cpic_advance_e(cpic_context *ctx, double px, double py, double pz,
+                   double dt_eps0);
+
+/* serial_update_ghosts on jfx,jfy,jfz (src/fields.h:102-271) followed by
+ * serial_update_ghosts_B on J (src/fields.h:11-100): the ghost update that
+ * advance_e performs first, exposed on its own. */
+int cpic_update_ghosts_j(cpic_context *ctx);
+
+/* EM_Field_Solver::e_energy / b_energy, src/fields.h:556-615: 0.5 * sum of
+ * squares over interior cells with dV = 1, accumulated in FP64.
+ * Synchronous. */
+int cpic_energies(cpic_context *ctx, double *e_energy, double *b_energy);
+
+/* Kinetic energy sum w me c^2 (gamma-1) in FP64 (the reference has no
+ * counterpart).  Synchronous. */
+int cpic_kinetic_energy(cpic_context *ctx, double me, double c, double *ke);
+
+/* Counting sort of the particles by cell (the commented-out
+ * Cabana::sortByKey of example/example.cpp:226-228).  The order of the
+ * particles within one cell is unspecified. */
+int cpic_sort_particles(cpic_context *ctx);
+
+/* nsteps iterations of the step loop body, example/example.cpp:216-279:
+ * [sort], load_interpolator, clear_accumulator, push, unload,
+ * advance_b(1/2), advance_e, advance_b(1/2), energies.  hist (may be NULL)
+ * receives one row of 3 doubles (E, B, kinetic) per step; entries of steps
+ * without the corresponding dump are left untouched.  The energies stay on
+ * the device until the loop ends (one synchronisation per call, where the
+ * reference synchronises at every dump).  Synchronous at return. */
+int cpic_step(cpic_context *ctx, const cpic_step_params *sp, int64_t nsteps,
+              double *hist);
+
+/* ---- diagnostics --------------------------------------------------------- */
+
+/* Stage timing with HIP events on the context stream.  While enabled, each
+ * stage launched by cpic_step (and by the single-stage entry points) is
+ * bracketed by events; cpic_stage_times returns, per stage, the number of
+ * timed launches and their summed milliseconds.  Stage ids: */
+#define CPIC_STAGE_INTERP 0
+#define CPIC_STAGE_CLEAR 1
+#define CPIC_STAGE_PUSH 2
+#define CPIC_STAGE_UNLOAD 3
+#define CPIC_STAGE_ADVANCE_B 4
+#define CPIC_STAGE_ADVANCE_E 5
+#define CPIC_STAGE_ENERGY 6
+#define CPIC_STAGE_SORT 7
+#define CPIC_NUM_STAGES 8
+int cpic_set_timing(cpic_context *ctx, int32_t enabled);
+int cpic_stage_times(cpic_context *ctx, int64_t *counts, double *ms);
+
+/* Particles that took the move_p branch in the last push, and the device
+ * error flags raised since creation (0 = none). */
+int cpic_last_push_movers(cpic_context *ctx, int64_t *movers);
+int cpic_device_errors(cpic_context *ctx, int32_t *flags);
+
+/* ---- deck support (host only, no device call) -------------------------- */
+
+/* Synthetic plasma loader for the decks of the benchmark and test
+ * configurations; it plays the role of a deck's Particle_Initializer::init
+ * (src/input/deck.h:79-158) for uniform thermal plasmas and beams.
+ * Particle p sits in interior cell c = p / nppc (x fastest).  From the
+ * counter-based stream r(p,k) = splitmix64(splitmix64(seed) + 8p + k):
+ * offsets x,y,z = 2u-1 with u = (r >> 11) 2^-53 (k = 0,1,2); three standard
+ * normals n0,n1 (Box-Muller on k = 3,4) and n2 (k = 5,6); momenta
+ * u_a = s d_a (1 + pert_amp sin(2 pi pert_mode X / len_x)) + uth_a n_a, where
+ * s = +1, or alternately +1/-1 for even/odd p when counter_streaming, and X
+ * is the particle's physical x position.  Arithmetic in double, stored as
+ * real_t.  n must equal nx*ny*nz*nppc. */
+typedef struct cpic_plasma_spec {
+  uint64_t seed;
+  int64_t nx, ny, nz, nppc;
+  double weight;
+  double uth[3];
+  double drift[3];
+  double pert_amp;
+  double len_x;
+  int32_t pert_mode;
+  int32_t counter_streaming;
+} cpic_plasma_spec;
+int cpic_init_plasma(const cpic_plasma_spec *spec, int32_t real_size,
+                     int64_t n, void *x, void *y, void *z, void *ux, void *uy,
+                     void *uz, void *w, int32_t *cell);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

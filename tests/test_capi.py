@@ -1,0 +1,64 @@
+"""CPU checks of the C-ABI boundary: the in-tree library loads, exports every
+symbol that include/cabanapic_amd.h declares, and the Python binding declares
+each of them.  No device call is made."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from cabanapic_amd import _capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for fn in os.listdir(inc):
+        if fn.endswith(".h"):
+            text = open(os.path.join(inc, fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(cpic_[a-z_0-9]+)\s*\(", text))
+    return names
+
+
+def test_header_declares_entry_points():
+    names = declared_symbols()
+    for n in ("cpic_create", "cpic_push", "cpic_load_interpolator_array",
+              "cpic_unload_accumulator_array", "cpic_advance_b", "cpic_advance_e",
+              "cpic_energies", "cpic_step"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_capi.LIB_PATH)
+    missing = [n for n in declared_symbols() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    assert declared_symbols() == set(_capi.SIGNATURES)
+
+
+def test_struct_layouts_match_header():
+    # cpic_config: 2 int32 + 4 int64; cpic_step_params: 15 doubles + 4 int32
+    assert ctypes.sizeof(_capi.Config) == 8 + 4 * 8
+    assert ctypes.sizeof(_capi.StepParams) == 15 * 8 + 4 * 4
+
+
+def test_last_error_without_device_call():
+    lib = _capi.load()
+    assert isinstance(lib.cpic_last_error(), bytes)
+
+
+def test_invalid_config_rejected_before_any_device_call():
+    lib = _capi.load()
+    cfg = _capi.Config(3, 1, 4, 4, 4, 10)
+    h = ctypes.c_void_p()
+    assert lib.cpic_create(ctypes.byref(cfg), ctypes.byref(h)) == _capi.CPIC_ERR_INVALID
+    assert b"real_size" in lib.cpic_last_error()
+    cfg = _capi.Config(4, 2, 4, 4, 4, 10)
+    assert lib.cpic_create(ctypes.byref(cfg), ctypes.byref(h)) == _capi.CPIC_ERR_UNSUPPORTED
+    with pytest.raises(_capi.CpicError):
+        _capi.check(lib.cpic_destroy(None) or _capi.CPIC_ERR_INVALID)

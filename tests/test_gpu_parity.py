@@ -1,0 +1,276 @@
+"""Parity of the HIP path (through the C-ABI) with the CPU oracle.
+
+Every stage is run on identical seeded inputs on both sides.  Per-cell and
+per-particle results must be bit-identical (cells, offsets, momenta,
+interpolator coefficients, J after unload, E and B after the Yee updates);
+accumulator sums, whose order differs (device atomics versus the oracle's
+serial order), must agree within the rounding bound of a reordered sum:
+|gpu - oracle| <= 64 eps sum|contributions| per slot.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+import states
+from cabanapic_amd import Simulation
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [np.float32, np.float64]
+GRIDS = [(6, 5, 4, 8), (1, 7, 1, 16), (9, 1, 3, 5), (3, 3, 3, 12)]
+
+
+def _name(dtype):
+    return "f32" if dtype == np.float32 else "f64"
+
+
+def sim_from_state(st):
+    sim = Simulation(st.nx, st.ny, st.nz, st.np, st.dtype)
+    sim.upload_fields(st.fields)
+    sim.upload_particles(st.p["x"], st.p["y"], st.p["z"], st.p["ux"], st.p["uy"], st.p["uz"],
+                         st.p["w"], st.cell)
+    return sim
+
+
+def assert_bitwise(a, b, what):
+    """Equality of the bit patterns (so -0.0 and 0.0 differ)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype, what
+    ui = {1: np.uint8, 4: np.uint32, 8: np.uint64}[a.dtype.itemsize]
+    diff = a.view(ui) != b.view(ui)
+    n = int(np.count_nonzero(diff))
+    first = np.flatnonzero(diff)[:5]
+    assert n == 0, f"{what}: {n} of {a.size} differ; first at {first}: {a.ravel()[first]} vs {b.ravel()[first]}"
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_load_interpolator_bitwise(dtype, grid):
+    nx, ny, nz, ppc = grid
+    st, _ = states.random_state(nx, ny, nz, 1, dtype, seed=11)
+    with sim_from_state(st) as sim:
+        sim.load_interpolator_array()
+        got = sim.download_interpolators()
+    st.load_interpolator()
+    assert_bitwise(got, st.interp, "interpolators")
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_push_particles_bitwise_and_accumulators(dtype, grid):
+    nx, ny, nz, ppc = grid
+    st, k = states.random_state(nx, ny, nz, ppc, dtype, seed=5 + nx)
+    st.load_interpolator()
+    with sim_from_state(st) as sim:
+        sim.load_interpolator_array()
+        sim.clear_accumulator_array()
+        sim.push(**k)
+        got = sim.download_particles()
+        acc = sim.download_accumulators()
+        movers = sim.last_push_movers()
+        assert sim.device_errors() == 0
+    absacc = np.zeros(st.acc.size)
+    st.clear_accumulator()
+    moved = st.push(**k, abs_acc=absacc)
+    assert moved > 0 and movers == moved
+    for m in ("x", "y", "z", "ux", "uy", "uz", "w"):
+        assert_bitwise(got[m], st.p[m], m)
+    assert_bitwise(got["cell"], st.cell, "cell")
+    eps = np.finfo(dtype).eps
+    tol = 64 * eps * absacc.reshape(acc.shape) + np.finfo(dtype).tiny
+    err = np.abs(acc.astype(np.float64) - st.acc.astype(np.float64))
+    assert (err <= tol).all(), float((err / tol).max())
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_push_with_random_interpolators(dtype):
+    """Interpolators uploaded directly (not derived from fields)."""
+    st, k = states.random_state(5, 4, 6, 10, dtype, seed=3)
+    rng = np.random.default_rng(4)
+    st.interp[:] = rng.uniform(-2, 2, st.interp.shape).astype(dtype)
+    with sim_from_state(st) as sim:
+        sim.upload_interpolators(st.interp)
+        assert_bitwise(sim.download_interpolators(), st.interp, "interp round trip")
+        sim.clear_accumulator_array()
+        sim.push(**k)
+        got = sim.download_particles()
+        acc = sim.download_accumulators()
+    absacc = np.zeros(st.acc.size)
+    st.clear_accumulator()
+    st.push(**k, abs_acc=absacc)
+    for m in ("x", "y", "z", "ux", "uy", "uz"):
+        assert_bitwise(got[m], st.p[m], m)
+    assert_bitwise(got["cell"], st.cell, "cell")
+    tol = 64 * np.finfo(dtype).eps * absacc.reshape(acc.shape) + np.finfo(dtype).tiny
+    assert (np.abs(acc.astype(np.float64) - st.acc) <= tol).all()
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_unload_bitwise(dtype, grid):
+    nx, ny, nz, _ = grid
+    st, _ = states.random_state(nx, ny, nz, 1, dtype, seed=21)
+    rng = np.random.default_rng(22)
+    st.acc[:] = rng.uniform(-1, 1, st.acc.shape).astype(dtype)
+    dx, dy, dz, dt = dtype(0.3), dtype(0.2), dtype(0.25), dtype(0.07)
+    with sim_from_state(st) as sim:
+        sim.upload_accumulators(st.acc)
+        sim.unload_accumulator_array(dx, dy, dz, dt)
+        got = sim.download_fields()
+    st.unload(dx, dy, dz, dt)
+    assert_bitwise(got, st.fields, "fields after unload")
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_ghost_update_j_bitwise(dtype, grid):
+    nx, ny, nz, _ = grid
+    st, _ = states.random_state(nx, ny, nz, 1, dtype, seed=31)
+    with sim_from_state(st) as sim:
+        sim.update_ghosts_j()
+        got = sim.download_fields()
+    st.fold_ghosts()
+    st.copy_ghosts(6)
+    assert_bitwise(got, st.fields, "fields after J ghost update")
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_advance_b_bitwise(dtype, grid):
+    nx, ny, nz, _ = grid
+    st, _ = states.random_state(nx, ny, nz, 1, dtype, seed=41)
+    p = (dtype(0.21), dtype(0.33), dtype(0.17))
+    with sim_from_state(st) as sim:
+        sim.advance_b(*p)
+        got = sim.download_fields()
+    st.advance_b(*p)
+    assert_bitwise(got, st.fields, "fields after advance_b")
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_advance_e_bitwise(dtype, grid):
+    nx, ny, nz, _ = grid
+    st, _ = states.random_state(nx, ny, nz, 1, dtype, seed=51)
+    p = (dtype(0.42), dtype(0.66), dtype(0.34))
+    dt_eps0 = dtype(0.09)
+    with sim_from_state(st) as sim:
+        sim.advance_e(*p, dt_eps0)
+        got = sim.download_fields()
+    st.advance_e(*p, dt_eps0)
+    assert_bitwise(got, st.fields, "fields after advance_e")
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_energies(dtype):
+    st, _ = states.random_state(7, 6, 5, 4, dtype, seed=61)
+    with sim_from_state(st) as sim:
+        e, b = sim.energies()
+        ke = sim.kinetic_energy()
+    e0, b0, _, _ = st.energies()
+    ke0 = st.kinetic_energy()
+    assert abs(e - e0) <= 1e-13 * e0
+    assert abs(b - b0) <= 1e-13 * b0
+    assert abs(ke - ke0) <= 1e-12 * ke0
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_one_step_particles_bitwise(dtype):
+    """A full fused step: the push sees identical interpolators, so the
+    particles match bit for bit; fields agree to the reordered-sum bound."""
+    nx, ny, nz, ppc = 6, 5, 4, 8
+    st, _ = states.random_state(nx, ny, nz, ppc, dtype, seed=71)
+    k = states.step_consts(dtype)
+    with sim_from_state(st) as sim:
+        hist = sim.step(dict(k, energy_interval=1), 1)
+        got = sim.download_particles()
+        f = sim.download_fields()
+    h, _ = st.run(states.consts_vector(k), 1)
+    for m in ("x", "y", "z", "ux", "uy", "uz"):
+        assert_bitwise(got[m], st.p[m], m)
+    assert_bitwise(got["cell"], st.cell, "cell")
+    scale = np.abs(st.fields).max()
+    tol = (1e-5 if dtype == np.float32 else 1e-13) * scale
+    assert np.abs(f.astype(np.float64) - st.fields).max() <= tol
+    assert abs(hist[0, 0] - h[0, 0]) <= 1e-5 * h[0, 0]
+    assert abs(hist[0, 1] - h[0, 1]) <= 1e-5 * h[0, 1]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_many_steps_energy_agreement(dtype):
+    """30 fused steps: trajectories drift apart only through the reordered J
+    sums; total field energy stays within a small relative band."""
+    nx, ny, nz, ppc = 8, 6, 5, 6
+    st, _ = states.random_state(nx, ny, nz, ppc, dtype, seed=81, umax=0.3)
+    st.fields[:] *= dtype(0.01)
+    k = states.step_consts(dtype)
+    with sim_from_state(st) as sim:
+        hist = sim.step(dict(k, energy_interval=1), 30)
+        got = sim.download_particles()
+        assert sim.device_errors() == 0
+    h, _ = st.run(states.consts_vector(k), 30)
+    assert got["cell"].size == st.cell.size
+    tot_g = hist[:, 0] + hist[:, 1]
+    tot_o = h[:, 0] + h[:, 1]
+    rel = np.abs(tot_g - tot_o) / tot_o
+    assert rel.max() < (1e-3 if dtype == np.float32 else 1e-9), rel.max()
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_sort_is_a_cell_ordered_permutation(dtype):
+    st, _ = states.random_state(7, 5, 6, 9, dtype, seed=91)
+    with sim_from_state(st) as sim:
+        sim.sort_particles()
+        got = sim.download_particles()
+    assert np.all(np.diff(got["cell"]) >= 0)
+    key = lambda d: np.lexsort([d[m] for m in ("x", "y", "z", "ux", "uy", "uz", "w")] +
+                               [d["cell"]])
+    src = dict(st.p, cell=st.cell)
+    a, b = key(src), key(got)
+    for m in ("x", "y", "z", "ux", "uy", "uz", "w", "cell"):
+        assert_bitwise(np.asarray(got[m])[b], np.asarray(src[m])[a], m)
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_sorted_push_matches_unsorted(dtype):
+    """Sorting only permutes particles: the pushed particle set and the
+    accumulators are those of the unsorted push."""
+    st, k = states.random_state(6, 6, 6, 10, dtype, seed=101)
+    st.load_interpolator()
+    with sim_from_state(st) as sim:
+        sim.sort_particles()
+        sim.load_interpolator_array()
+        sim.clear_accumulator_array()
+        sim.push(**k)
+        got = sim.download_particles()
+        acc = sim.download_accumulators()
+    absacc = np.zeros(st.acc.size)
+    st.clear_accumulator()
+    st.push(**k, abs_acc=absacc)
+    order_g = np.lexsort([got[m] for m in ("uz", "uy", "ux")])
+    order_o = np.lexsort([st.p[m] for m in ("uz", "uy", "ux")])
+    for m in ("x", "y", "z", "ux", "uy", "uz"):
+        assert_bitwise(got[m][order_g], st.p[m][order_o], m)
+    assert_bitwise(got["cell"][order_g], st.cell[order_o], "cell")
+    tol = 64 * np.finfo(dtype).eps * absacc.reshape(acc.shape) + np.finfo(dtype).tiny
+    assert (np.abs(acc.astype(np.float64) - st.acc) <= tol).all()
+
+
+def test_invalid_cell_rejected_on_upload():
+    st, _ = states.random_state(4, 4, 4, 2, np.float32, seed=1)
+    sim = Simulation(4, 4, 4, st.np, np.float32)
+    bad = st.cell.copy()
+    bad[3] = 0  # a ghost voxel
+    with pytest.raises(Exception):
+        sim.upload_particles(st.p["x"], st.p["y"], st.p["z"], st.p["ux"], st.p["uy"],
+                             st.p["uz"], st.p["w"], bad)
+    sim.close()
+
+
+def test_reflect_boundary_unsupported():
+    from cabanapic_amd import CpicError
+    st, k = states.random_state(4, 4, 4, 2, np.float32, seed=1)
+    with sim_from_state(st) as sim:
+        with pytest.raises(CpicError):
+            sim.push(**k, boundary=0)
