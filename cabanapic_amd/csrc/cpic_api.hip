@@ -6,6 +6,7 @@
 // not match its buffers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -14,7 +15,7 @@
 #include <vector>
 
 #include "../../include/cabanapic_amd.h"
-#include "cpic_kernels.h"
+#include "cpic_push_kernels.h"
 
 using namespace cpic;
 
@@ -50,24 +51,38 @@ struct cpic_context {
   int real_size = 4;
   long nx = 0, ny = 0, nz = 0;
   long ncells = 0;  // Ng
-  long cap = 0;     // particle capacity
-  long np = 0;      // particles resident
+  long nc = 0;      // interior cells
+  long nc_pad = 0;  // interior cells rounded up to whole groups of 64
+  long cap = 0;     // particle capacity declared at creation
+  long np = 0;      // particles resident (conserved by the push)
   Grid grid{};
+  CellGrid cg{};
 
-  // device arrays
-  void* part[7] = {};     // x,y,z,ux,uy,uz,w
-  int* cell = nullptr;
-  void* alt[7] = {};      // sort destination (allocated on first sort)
-  int* alt_cell = nullptr;
+  // particle columns (double-buffered), see cpic_push_kernels.h
+  int D = 0;               // rows per column
+  void* cols[2][7] = {};   // x,y,z,ux,uy,uz,w
+  int* cnt[2] = {};        // particles per interior cell
+  int cur = 0;             // buffer holding the current particles
+  // crossing records: list 1 (first and third crossings), list 2 (second)
+  int M1 = 0, M2 = 0;
+  void* rec1 = nullptr;
+  void* rec2 = nullptr;
+  int* rc1 = nullptr;
+  int* rc2 = nullptr;
+  void* ovf_rec = nullptr;
+  int* ovf_cell = nullptr;
+  unsigned int* ovf_count = nullptr;
+  int ovf_cap = 0;
+  int* max_count = nullptr;  // largest column occupancy seen by the last push
+
   void* fields = nullptr;
   void* interp = nullptr;
-  void* acc = nullptr;
+  void* acc = nullptr;             // slot-major [12][Ng]
+  bool acc_ghosts_zero = true;     // ghost accumulators known to be zero
   double* partials = nullptr;  // reduction partials
   double* red_out = nullptr;   // [E, B, KE]
   int* err = nullptr;
   unsigned int* movers = nullptr;
-  unsigned int* sort_counts = nullptr;
-  unsigned int* sort_sums = nullptr;
   double* hist_dev = nullptr;  // per-step energy rows of cpic_step
   long hist_cap = 0;           // rows allocated in hist_dev
   int n_partials = 0;
@@ -87,6 +102,7 @@ struct cpic_context {
   double stage_ms[CPIC_NUM_STAGES] = {};
 
   size_t rsz() const { return (size_t)real_size; }
+  size_t col_len() const { return (size_t)nc_pad * (size_t)D; }
 };
 
 namespace {
@@ -178,37 +194,82 @@ int stage_clear(cpic_context* c) {
   StageTimer t(c, CPIC_STAGE_CLEAR);
   int s = fill_zero<R>(c, c->acc, (size_t)c->ncells * 12);
   if (s) return s;
+  c->acc_ghosts_zero = true;
   return t.finish();
 }
 
 template <class R>
-int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp) {
+ColState<R> col_state(cpic_context* c, int from) {
+  ColState<R> cs;
+  for (int k = 0; k < 7; ++k) {
+    cs.in[k] = (const R*)c->cols[from][k];
+    cs.out[k] = (R*)c->cols[1 - from][k];
+  }
+  cs.cnt_in = c->cnt[from];
+  cs.cnt_out = c->cnt[1 - from];
+  cs.D = c->D;
+  return cs;
+}
+
+// push + move_p + deposit: pass 1, three finishing passes, spill fallback.
+// accumulate = 1 adds to the accumulators as the reference's push does;
+// accumulate = 0 overwrites them (valid right after a clear).
+template <class R>
+int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, int accumulate) {
   StageTimer t(c, CPIC_STAGE_PUSH);
-  if (int z = fill_zero<unsigned int>(c, c->movers, 1)) return z;
-  if (c->np > 0) {
-    PushArgs<R> a;
-    a.x = (R*)c->part[0];
-    a.y = (R*)c->part[1];
-    a.z = (R*)c->part[2];
-    a.ux = (R*)c->part[3];
-    a.uy = (R*)c->part[4];
-    a.uz = (R*)c->part[5];
-    a.w = (const R*)c->part[6];
-    a.cell = c->cell;
-    a.interp = (const R*)c->interp;
-    a.acc = (R*)c->acc;
-    a.np = (int)c->np;
-    a.g = c->grid;
-    a.qdt_2mc = qdt_2mc;
-    a.cdt_dx = cdt_dx;
-    a.cdt_dy = cdt_dy;
-    a.cdt_dz = cdt_dz;
-    a.qsp = qsp;
-    a.err = c->err;
-    a.movers = c->movers;
-    k_push<R><<<cdiv(c->np, kBlock), kBlock, 0, c->stream>>>(a);
+  int s;
+  if ((s = fill_zero<unsigned int>(c, c->movers, 1))) return s;
+  if (c->np == 0 || c->D == 0) return t.finish();
+  if ((s = fill_zero<unsigned int>(c, c->ovf_count, 1))) return s;
+  if ((s = fill_zero<int>(c, c->max_count, 1))) return s;
+  const int blocks = cdiv(c->nc, kBlock);
+  Overflow<R> ovf{(R*)c->ovf_rec, c->ovf_cell, c->ovf_count, c->ovf_cap};
+  RecordList<R> L1{(R*)c->rec1, c->rc1, c->M1};
+  RecordList<R> L2{(R*)c->rec2, c->rc2, c->M2};
+  const ColState<R> cs = col_state<R>(c, c->cur);
+
+  PushPass<R> p;
+  p.cs = cs;
+  p.out = L1;
+  p.ovf = ovf;
+  p.interp = (const R*)c->interp;
+  p.acc = (R*)c->acc;
+  p.g = c->cg;
+  p.ncells = (int)c->ncells;
+  p.qdt_2mc = qdt_2mc;
+  p.cdt_dx = cdt_dx;
+  p.cdt_dy = cdt_dy;
+  p.cdt_dz = cdt_dz;
+  p.qsp = qsp;
+  p.accumulate = accumulate;
+  p.err = c->err;
+  p.movers = c->movers;
+  p.max_count = c->max_count;
+  k_push_pass<R><<<blocks, kBlock, 0, c->stream>>>(p);
+  CHECK_LAUNCH();
+
+  // the streak after the 1st, 2nd and 3rd crossings
+  const RecordList<R> ins[3] = {L1, L2, L1};
+  const RecordList<R> outs[3] = {L2, L1, L2};
+  for (int pass = 0; pass < 3; ++pass) {
+    FinishPass<R> f;
+    f.cs = cs;
+    f.in = ins[pass];
+    f.out = outs[pass];
+    f.ovf = ovf;
+    f.acc = (R*)c->acc;
+    f.g = c->cg;
+    f.ncells = (int)c->ncells;
+    f.qsp = qsp;
+    f.last = (pass == 2) ? 1 : 0;
+    f.err = c->err;
+    k_finish_pass<R><<<blocks, kBlock, 0, c->stream>>>(f);
     CHECK_LAUNCH();
   }
+  SpillPass<R> sp{cs, ovf, (R*)c->acc, c->cg, (int)c->ncells, qsp, c->err};
+  k_spill_pass<R><<<64, kBlock, 0, c->stream>>>(sp);
+  CHECK_LAUNCH();
+  c->cur = 1 - c->cur;
   return t.finish();
 }
 
@@ -289,59 +350,65 @@ int launch_field_energy(cpic_context* c, double* out) {
 
 template <class R>
 int launch_kinetic(cpic_context* c, double scale, double* out) {
-  const int blocks = std::max(1, std::min(cdiv(std::max(c->np, 1L), kBlock), c->n_partials));
-  k_kinetic_partial<R><<<blocks, kBlock, 0, c->stream>>>(
-      (const R*)c->part[3], (const R*)c->part[4], (const R*)c->part[5], (const R*)c->part[6],
-      (int)c->np, c->partials + 2 * (size_t)c->n_partials);
+  double* part = c->partials + 2 * (size_t)c->n_partials;
+  int blocks = 1;
+  if (c->D > 0 && c->np > 0) {
+    blocks = std::max(1, std::min(cdiv(c->nc, kBlock), c->n_partials));
+    const ColState<R> cs = col_state<R>(c, c->cur);
+    k_kinetic_columns<R><<<blocks, kBlock, 0, c->stream>>>(cs, (int)c->nc, part);
+  } else {
+    k_fill_zero<double><<<1, kBlock, 0, c->stream>>>(part, 1);
+  }
   CHECK_LAUNCH();
-  k_reduce_final<1><<<1, kBlock, 0, c->stream>>>(c->partials + 2 * (size_t)c->n_partials, blocks,
-                                                 scale, 0.0, out);
+  k_reduce_final<1><<<1, kBlock, 0, c->stream>>>(part, blocks, scale, 0.0, out);
   CHECK_LAUNCH();
   return CPIC_OK;
 }
 
-template <class R>
-int stage_sort(cpic_context* c) {
-  StageTimer t(c, CPIC_STAGE_SORT);
-  if (c->np == 0) return t.finish();
-  if (!c->alt_cell) {
+// (Re)allocates the two particle column buffers (D rows per cell) and the
+// crossing record lists.  Capacities: D = max(2.5 x, +48) the largest column
+// at upload; M1 record slots per (cell, face) for the common first
+// crossings, M2 for the rarer later ones; spills beyond them are finished by
+// the fallback kernel.
+int ensure_columns(cpic_context* c, int D, int M1, int M2) {
+  if (c->D == D && c->M1 == M1 && c->M2 == M2 && c->cnt[0]) return CPIC_OK;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  for (int b = 0; b < 2; ++b) {
     for (int k = 0; k < 7; ++k) {
-      int s = alloc(&c->alt[k], c->rsz() * c->cap);
-      if (s) return s;
+      if (c->cols[b][k]) HIP_TRY(hipFree(c->cols[b][k]));
+      c->cols[b][k] = nullptr;
     }
-    int s = alloc((void**)&c->alt_cell, sizeof(int) * c->cap);
-    if (s) return s;
+    if (c->cnt[b]) HIP_TRY(hipFree(c->cnt[b]));
+    c->cnt[b] = nullptr;
   }
-  const long ng = c->ncells;
-  const int nsum = cdiv(ng, kBlock * 4);
-  int s = fill_zero<unsigned int>(c, c->sort_counts, (size_t)ng);
-  if (s) return s;
-  k_sort_histogram<<<cdiv(c->np, kBlock), kBlock, 0, c->stream>>>(c->cell, (int)c->np,
-                                                                   c->sort_counts, (int)ng, c->err);
-  CHECK_LAUNCH();
-  // in-place exclusive scan of the counts: counts -> offsets
-  k_scan_blocks<<<nsum, kBlock, 0, c->stream>>>(c->sort_counts, c->sort_counts, (int)ng,
-                                                c->sort_sums);
-  CHECK_LAUNCH();
-  k_scan_sums<<<1, kBlock, 0, c->stream>>>(c->sort_sums, nsum);
-  CHECK_LAUNCH();
-  k_scan_add<<<cdiv(ng, kBlock), kBlock, 0, c->stream>>>(c->sort_counts, (int)ng, c->sort_sums);
-  CHECK_LAUNCH();
-  SortArgs<R> a;
-  for (int k = 0; k < 7; ++k) {
-    a.src[k] = (const R*)c->part[k];
-    a.dst[k] = (R*)c->alt[k];
+  void** recs[] = {&c->rec1, &c->rec2, (void**)&c->rc1, (void**)&c->rc2, &c->ovf_rec,
+                   (void**)&c->ovf_cell};
+  for (void** p : recs) {
+    if (*p) HIP_TRY(hipFree(*p));
+    *p = nullptr;
   }
-  a.src_cell = c->cell;
-  a.dst_cell = c->alt_cell;
-  a.cursor = c->sort_counts;
-  a.np = (int)c->np;
-  a.ncells = (int)ng;
-  k_sort_scatter<R><<<cdiv(c->np, kBlock), kBlock, 0, c->stream>>>(a);
-  CHECK_LAUNCH();
-  for (int k = 0; k < 7; ++k) std::swap(c->part[k], c->alt[k]);
-  std::swap(c->cell, c->alt_cell);
-  return t.finish();
+  c->D = 0;
+  const size_t len = (size_t)c->nc_pad * (size_t)D;
+  int s;
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < 7; ++k)
+      if ((s = alloc(&c->cols[b][k], c->rsz() * len))) return s;
+    if ((s = alloc((void**)&c->cnt[b], sizeof(int) * c->nc_pad))) return s;
+  }
+  const size_t per = (size_t)kRecordLen * c->rsz() * 6 * (size_t)c->nc;
+  if ((s = alloc(&c->rec1, per * M1))) return s;
+  if ((s = alloc(&c->rec2, per * M2))) return s;
+  if ((s = alloc((void**)&c->rc1, sizeof(int) * 6 * c->nc))) return s;
+  if ((s = alloc((void**)&c->rc2, sizeof(int) * 6 * c->nc))) return s;
+  c->ovf_cap = (int)std::max(65536L, c->nc / 4);
+  if ((s = alloc(&c->ovf_rec, (size_t)kRecordLen * c->rsz() * c->ovf_cap))) return s;
+  if ((s = alloc((void**)&c->ovf_cell, sizeof(int) * c->ovf_cap))) return s;
+  if ((s = fill_zero<int>(c, c->cnt[0], c->nc_pad))) return s;
+  if ((s = fill_zero<int>(c, c->cnt[1], c->nc_pad))) return s;
+  c->D = D;
+  c->M1 = M1;
+  c->M2 = M2;
+  return CPIC_OK;
 }
 
 int check_periodic(int boundary) {
@@ -397,6 +464,14 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
   c->grid.sy = (int)X;
   c->grid.sz = (int)(X * Y);
   c->grid.ncells = (int)c->ncells;
+  c->nc = cfg->nx * cfg->ny * cfg->nz;
+  c->nc_pad = (c->nc + 63) / 64 * 64;
+  c->cg.nx = (int)c->nx;
+  c->cg.ny = (int)c->ny;
+  c->cg.nz = (int)c->nz;
+  c->cg.nc = (int)c->nc;
+  c->cg.sy = (int)X;
+  c->cg.sz = (int)(X * Y);
   c->n_partials = 4096;
 
   auto bail = [&](int s) {
@@ -407,9 +482,6 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
   if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(CPIC_ERR_HIP, "no HIP device"));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(CPIC_ERR_HIP, "hipStreamCreate failed"));
-  for (int k = 0; k < 7; ++k)
-    if ((s = alloc(&c->part[k], c->rsz() * c->cap))) return bail(s);
-  if ((s = alloc((void**)&c->cell, sizeof(int) * c->cap))) return bail(s);
   if ((s = alloc(&c->fields, c->rsz() * 9 * c->ncells))) return bail(s);
   if ((s = alloc(&c->interp, c->rsz() * kInterpStride * c->ncells))) return bail(s);
   if ((s = alloc(&c->acc, c->rsz() * 12 * c->ncells))) return bail(s);
@@ -417,9 +489,8 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
   if ((s = alloc((void**)&c->red_out, sizeof(double) * 4))) return bail(s);
   if ((s = alloc((void**)&c->err, sizeof(int)))) return bail(s);
   if ((s = alloc((void**)&c->movers, sizeof(unsigned int)))) return bail(s);
-  if ((s = alloc((void**)&c->sort_counts, sizeof(unsigned int) * c->ncells))) return bail(s);
-  if ((s = alloc((void**)&c->sort_sums, sizeof(unsigned int) * (cdiv(c->ncells, kBlock * 4) + 1))))
-    return bail(s);
+  if ((s = alloc((void**)&c->ovf_count, sizeof(unsigned int)))) return bail(s);
+  if ((s = alloc((void**)&c->max_count, sizeof(int)))) return bail(s);
   s = with_real(c, [&](auto r) {
     using R = decltype(r);
     int t;
@@ -446,12 +517,15 @@ int cpic_destroy(cpic_context* c) {
     (void)hipEventDestroy(p.b);
   }
   for (auto e : c->free_events) (void)hipEventDestroy(e);
-  for (int k = 0; k < 7; ++k) {
-    if (c->part[k]) (void)hipFree(c->part[k]);
-    if (c->alt[k]) (void)hipFree(c->alt[k]);
+  for (int b = 0; b < 2; ++b) {
+    for (int k = 0; k < 7; ++k)
+      if (c->cols[b][k]) (void)hipFree(c->cols[b][k]);
+    if (c->cnt[b]) (void)hipFree(c->cnt[b]);
   }
-  void* ptrs[] = {c->cell,   c->alt_cell, c->fields,      c->interp,    c->acc,     c->partials,
-                  c->red_out, c->err,      c->movers,      c->sort_counts, c->sort_sums, c->hist_dev};
+  void* ptrs[] = {c->rec1,   c->rec2,     c->rc1,      c->rc2,     c->ovf_rec,
+                  c->ovf_cell, c->ovf_count, c->max_count, c->fields, c->interp,
+                  c->acc,    c->partials, c->red_out,  c->err,     c->movers,
+                  c->hist_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -486,17 +560,38 @@ int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void*
   }
   // every particle must sit in an interior voxel (src/types.h:40)
   const long X = c->nx + 2, Y = c->ny + 2;
+  std::vector<int> ci(n);
+  std::vector<int> count(c->nc_pad, 0);
+  int maxc = 0;
   for (int64_t p = 0; p < n; ++p) {
     const long v = cell[p];
     const long ix = v % X, iy = (v / X) % Y, iz = v / (X * Y);
     if (v < 0 || v >= c->ncells || ix < 1 || ix > c->nx || iy < 1 || iy > c->ny || iz < 1 ||
         iz > c->nz)
       return fail(CPIC_ERR_INVALID, "particle %lld has non-interior cell %ld", (long long)p, v);
+    ci[p] = (int)((ix - 1) + c->nx * ((iy - 1) + c->ny * (iz - 1)));
+    maxc = std::max(maxc, ++count[ci[p]]);
+  }
+  auto up8 = [](int v) { return (v + 7) / 8 * 8; };
+  const int D = up8(std::max({maxc * 5 / 2, maxc + 48, 16}));
+  const int M1 = std::min(D, std::max(16, up8(maxc / 2)));
+  const int M2 = std::min(D, 8);
+  if (int s = ensure_columns(c, D, M1, M2)) return s;
+  // bin on the host, keeping the input order within each cell
+  const size_t rs = c->rsz(), len = c->col_len();
+  std::vector<unsigned char> buf(rs * len * 7, 0);
+  std::vector<int> fill(c->nc_pad, 0);
+  for (int64_t p = 0; p < n; ++p) {
+    const int cc = ci[p];
+    const size_t slot = ((size_t)(cc >> 6) * D + (size_t)fill[cc]++) * 64 + (cc & 63);
+    for (int k = 0; k < 7; ++k)
+      memcpy(&buf[(k * len + slot) * rs], (const unsigned char*)src[k] + p * rs, rs);
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int k = 0; k < 7; ++k)
-    if (n > 0) HIP_TRY(hipMemcpy(c->part[k], src[k], c->rsz() * n, hipMemcpyHostToDevice));
-  if (n > 0) HIP_TRY(hipMemcpy(c->cell, cell, sizeof(int) * n, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->cols[0][k], &buf[k * len * rs], rs * len, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->cnt[0], count.data(), sizeof(int) * c->nc_pad, hipMemcpyHostToDevice));
+  c->cur = 0;
   c->np = n;
   return CPIC_OK;
 }
@@ -506,11 +601,40 @@ int cpic_download_particles(cpic_context* c, int64_t n, void* x, void* y, void* 
   if (int s = check_ctx(c)) return s;
   if (n != c->np)
     return fail(CPIC_ERR_INVALID, "download of %lld particles, %ld resident", (long long)n, c->np);
+  if (n == 0) return CPIC_OK;
   void* dst[7] = {x, y, z, ux, uy, uz, w};
+  const size_t rs = c->rsz(), len = c->col_len();
+  std::vector<int> count(c->nc_pad);
+  std::vector<unsigned char> buf(rs * len);
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < 7; ++k)
-    if (dst[k] && n > 0) HIP_TRY(hipMemcpy(dst[k], c->part[k], c->rsz() * n, hipMemcpyDeviceToHost));
-  if (cell && n > 0) HIP_TRY(hipMemcpy(cell, c->cell, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(count.data(), c->cnt[c->cur], sizeof(int) * c->nc_pad, hipMemcpyDeviceToHost));
+  long total = 0;
+  for (long i = 0; i < c->nc; ++i) total += count[i];
+  if (total != n)
+    return fail(CPIC_ERR_DEVICE, "columns hold %ld particles, %lld expected", total, (long long)n);
+  const long X = c->nx + 2, Y = c->ny + 2;
+  for (int k = -1; k < 7; ++k) {
+    if (k >= 0) {
+      if (!dst[k]) continue;
+      HIP_TRY(hipMemcpy(buf.data(), c->cols[c->cur][k], rs * len, hipMemcpyDeviceToHost));
+    } else if (!cell) {
+      continue;
+    }
+    // cells in interior order (x fastest), rows in column order
+    long o = 0;
+    for (long i = 0; i < c->nc; ++i) {
+      const long ix = i % c->nx, iy = (i / c->nx) % c->ny, iz = i / (c->nx * c->ny);
+      const int v = (int)((ix + 1) + X * ((iy + 1) + Y * (iz + 1)));
+      for (int r = 0; r < count[i]; ++r, ++o) {
+        if (k < 0) {
+          cell[o] = v;
+        } else {
+          const size_t slot = ((size_t)(i >> 6) * c->D + (size_t)r) * 64 + (i & 63);
+          memcpy((unsigned char*)dst[k] + o * rs, &buf[slot * rs], rs);
+        }
+      }
+    }
+  }
   return CPIC_OK;
 }
 
@@ -569,16 +693,29 @@ int cpic_download_interpolators(cpic_context* c, void* out) {
 int cpic_upload_accumulators(cpic_context* c, const void* in) {
   if (int s = check_ctx(c)) return s;
   if (!in) return fail(CPIC_ERR_INVALID, "null accumulators");
+  const size_t rs = c->rsz();
+  std::vector<unsigned char> buf(rs * 12 * c->ncells);
+  const unsigned char* src = (const unsigned char*)in;
+  for (long i = 0; i < c->ncells; ++i)
+    for (int k = 0; k < 12; ++k)
+      memcpy(&buf[((size_t)k * c->ncells + i) * rs], &src[((size_t)i * 12 + k) * rs], rs);
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(c->acc, in, c->rsz() * 12 * c->ncells, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->acc, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  c->acc_ghosts_zero = false;
   return CPIC_OK;
 }
 
 int cpic_download_accumulators(cpic_context* c, void* out) {
   if (int s = check_ctx(c)) return s;
   if (!out) return fail(CPIC_ERR_INVALID, "null accumulators");
+  const size_t rs = c->rsz();
+  std::vector<unsigned char> buf(rs * 12 * c->ncells);
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(out, c->acc, c->rsz() * 12 * c->ncells, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(buf.data(), c->acc, buf.size(), hipMemcpyDeviceToHost));
+  unsigned char* dst = (unsigned char*)out;
+  for (long i = 0; i < c->ncells; ++i)
+    for (int k = 0; k < 12; ++k)
+      memcpy(&dst[((size_t)i * 12 + k) * rs], &buf[((size_t)k * c->ncells + i) * rs], rs);
   return CPIC_OK;
 }
 
@@ -598,7 +735,7 @@ int cpic_push(cpic_context* c, double qdt_2mc, double cdt_dx, double cdt_dy, dou
   if (int s = check_periodic(boundary)) return s;
   return with_real(c, [&](auto r) {
     using R = decltype(r);
-    return stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp);
+    return stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1);
   });
 }
 
@@ -657,8 +794,9 @@ int cpic_kinetic_energy(cpic_context* c, double me, double cc, double* ke) {
 }
 
 int cpic_sort_particles(cpic_context* c) {
-  if (int s = check_ctx(c)) return s;
-  return with_real(c, [&](auto r) { return stage_sort<decltype(r)>(c); });
+  // The particles are binned by cell at upload and stay binned: every push
+  // writes each particle into the column of the cell it ends in.
+  return check_ctx(c);
 }
 
 int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, double* hist) {
@@ -687,12 +825,12 @@ int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, doubl
     const R hpx = half * px, hpy = half * py, hpz = half * pz;
     for (int64_t step = 1; step <= nsteps; ++step) {
       int t;
-      if (sp->sort_interval > 0 && (step - 1) % sp->sort_interval == 0)
-        if ((t = stage_sort<R>(c))) return t;
       if ((t = stage_load_interpolator<R>(c))) return t;
-      if ((t = stage_clear<R>(c))) return t;
+      // the push overwrites every interior accumulator, so the clear is only
+      // needed while ghost accumulators may hold data
+      if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
       if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
-                             (R)sp->qsp)))
+                             (R)sp->qsp, 0)))
         return t;
       if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;
       if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;

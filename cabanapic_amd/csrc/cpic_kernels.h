@@ -10,12 +10,12 @@
 // atomic as well).
 //
 // Device layout in HBM:
-//   particles      SoA: x,y,z,ux,uy,uz,w (R) + cell (int32)
 //   fields         9 x Ng (R), component-major (as the host layout)
 //   interpolators  Ng x 20 (R), cell-major: the 18 coefficients of
 //                  src/types.h:64-84 padded to 20 so a cell is five 4-wide
 //                  vector loads
-//   accumulators   Ng x 12 (R), cell-major jx[4] jy[4] jz[4]
+//   accumulators   12 x Ng (R), slot-major: jx[0..3], jy[0..3], jz[0..3]
+//   particles      per-cell columns, see cpic_push_kernels.h
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,8 +31,10 @@ constexpr int kBlock = 256;
 constexpr int kMaxStreaks = 16;
 
 enum DeviceError : int {
-  kErrBadCell = 1,      // particle cell outside the voxel array
-  kErrStreakLimit = 2,  // move_p did not finish within kMaxStreaks
+  kErrBadCell = 1,       // particle cell outside the voxel array
+  kErrStreakLimit = 2,   // move_p did not finish within kMaxStreaks
+  kErrColumnFull = 4,    // a cell received more particles than its column holds
+  kErrOverflowFull = 8,  // the spill list of crossing records is full
 };
 
 template <class R> struct Vec4T;
@@ -125,225 +127,9 @@ __global__ void __launch_bounds__(kBlock) k_fill_zero(R* __restrict__ p, size_t 
 }
 
 // ---------------------------------------------------------------------------
-// The push (src/push.h:65-295) with move_p (src/move_p.h:59-374).
-
-template <class R> struct PushArgs {
-  R* __restrict__ x;
-  R* __restrict__ y;
-  R* __restrict__ z;
-  R* __restrict__ ux;
-  R* __restrict__ uy;
-  R* __restrict__ uz;
-  const R* __restrict__ w;
-  int* __restrict__ cell;
-  const R* __restrict__ interp;
-  R* __restrict__ acc;
-  int np;
-  Grid g;
-  R qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
-  int* err;
-  unsigned int* movers;
-};
-
-// The correction term of a streak in move_p: a real_t product scaled by the
-// double constant 1/3 and rounded back (src/move_p.h:154).
-__device__ __forceinline__ float streak_v5(float q, float a, float b, float c) {
-  return (float)((double)(q * a * b * c) * (1. / 3.));
-}
-__device__ __forceinline__ double streak_v5(double q, double a, double b, double c) {
-  return q * a * b * c * (1. / 3.);
-}
-
-// 1/sqrt argument handling of push.h:148,169: sqrtf narrows a double
-// argument to float in the double build.
-__device__ __forceinline__ float sqrtf_narrow(float v) { return sqrtf(v); }
-__device__ __forceinline__ double sqrtf_narrow(double v) { return (double)sqrtf((float)v); }
-
-// One current component's four quadrant contributions for a streak with
-// displacement dA along the component axis and midpoint (mB, mC) on the two
-// transverse axes (push.h:218-232, move_p.h:156-170), added atomically.
-template <class R>
-__device__ __forceinline__ void deposit_component(R* a, R q, R dA, R mB, R mC, R v5) {
-  R t = q * dA;
-  R s1 = t * mB;
-  R s0 = t - s1;
-  s1 += t;
-  R c = R(1) + mC;
-  const R s2 = s0 * c;
-  const R s3 = s1 * c;
-  c = R(1) - mC;
-  s0 *= c;
-  s1 *= c;
-  s0 += v5;
-  s1 -= v5;
-  unsafeAtomicAdd(a + 0, s0);
-  unsafeAtomicAdd(a + 1, s1);
-  unsafeAtomicAdd(a + 2, s2 - v5);
-  unsafeAtomicAdd(a + 3, s3 + v5);
-}
-
-template <class R>
-__device__ __forceinline__ void deposit_streak(R* acc, int cell, R q, R dx, R dy, R dz, R mx,
-                                               R my, R mz, R v5) {
-  R* a = acc + (size_t)cell * 12;
-  deposit_component(a + 0, q, dx, my, mz, v5);
-  deposit_component(a + 4, q, dy, mz, mx, v5);
-  deposit_component(a + 8, q, dz, mx, my, v5);
-}
-
-// move_p: split the remaining track at cell faces, deposit each streak,
-// periodic wrap at the domain faces.  pos/rem are in/out.
-template <class R>
-__device__ void move_p(const PushArgs<R>& a, int p, R pos[3], int& cell, R q, R rem[3]) {
-  const Grid& g = a.g;
-  for (int it = 0;; ++it) {
-    if (it == kMaxStreaks) {
-      atomicOr(a.err, kErrStreakLimit);
-      break;
-    }
-    R dir[3], len3[3], dsp[3], mid[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      dsp[k] = rem[k];
-      mid[k] = pos[k];
-      dir[k] = (dsp[k] > 0) ? R(1) : R(-1);
-      len3[k] = (dsp[k] == 0) ? R(3.4e38) : (dir[k] - mid[k]) / dsp[k];
-    }
-    R len = 2;
-    int axis = 3;
-    if (len3[0] < len) len = len3[0], axis = 0;
-    if (len3[1] < len) len = len3[1], axis = 1;
-    if (len3[2] < len) len = len3[2], axis = 2;
-    len *= R(0.5);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      dsp[k] *= len;
-      mid[k] += dsp[k];
-    }
-    const R v5 = streak_v5(q, dsp[0], dsp[1], dsp[2]);
-    deposit_streak(a.acc, cell, q, dsp[0], dsp[1], dsp[2], mid[0], mid[1], mid[2], v5);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      rem[k] -= dsp[k];
-      pos[k] += dsp[k] + dsp[k];
-    }
-    if (axis == 3) break;
-
-    const R side = dir[axis];
-    // decode the voxel, step across the face, wrap periodically
-    int ix = cell % g.sy;
-    const int rest = cell / g.sy;
-    int iy = rest % (g.ny + 2);
-    int iz = rest / (g.ny + 2);
-    const int step = (side > 0) ? 1 : -1;
-    if (axis == 0) {
-      ix += step;
-      if (ix == 0) ix = g.nx;
-      else if (ix == g.nx + 1) ix = 1;
-    } else if (axis == 1) {
-      iy += step;
-      if (iy == 0) iy = g.ny;
-      else if (iy == g.ny + 1) iy = 1;
-    } else {
-      iz += step;
-      if (iz == 0) iz = g.nz;
-      else if (iz == g.nz + 1) iz = 1;
-    }
-    cell = voxel(g, ix, iy, iz);
-    if ((unsigned)cell >= (unsigned)g.ncells) {
-      atomicOr(a.err, kErrBadCell);
-      cell = 0;
-      break;
-    }
-    pos[axis] = -side;
-  }
-}
-
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_push(PushArgs<R> a) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.np) return;
-  int ii = a.cell[p];
-  if ((unsigned)ii >= (unsigned)a.g.ncells) {
-    atomicOr(a.err, kErrBadCell);
-    return;
-  }
-  typedef typename Vec4T<R>::type V4;
-  const V4* f = reinterpret_cast<const V4*>(a.interp + (size_t)ii * kInterpStride);
-  const V4 e0 = f[0], e1 = f[1], e2 = f[2], b0 = f[3], b1 = f[4];
-  const R one = 1.;
-  const R one_third = 1. / 3.;
-  const R two_fifteenths = 2. / 15.;
-  const R qdt_2mc = a.qdt_2mc;
-
-  const R dx = a.x[p], dy = a.y[p], dz = a.z[p];
-  // field gather, push.h:124-138
-  const R hax = qdt_2mc * ((e0.x + dy * e0.y) + dz * (e0.z + dy * e0.w));
-  const R hay = qdt_2mc * ((e1.x + dz * e1.y) + dx * (e1.z + dz * e1.w));
-  const R haz = qdt_2mc * ((e2.x + dx * e2.y) + dy * (e2.z + dx * e2.w));
-  const R cbx = b0.x + dx * b0.y;
-  const R cby = b0.z + dy * b0.w;
-  const R cbz = b1.x + dz * b1.y;
-  R ux = a.ux[p], uy = a.uy[p], uz = a.uz[p];
-  ux += hax;
-  uy += hay;
-  uz += haz;
-  // Boris rotation with the tan series, push.h:148-163
-  R v0 = qdt_2mc / sqrtf_narrow(one + (ux * ux + (uy * uy + uz * uz)));
-  R v1 = cbx * cbx + (cby * cby + cbz * cbz);
-  R v2 = (v0 * v0) * v1;
-  R v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
-  R v4 = v3 / (one + v1 * (v3 * v3));
-  v4 += v4;
-  v0 = ux + v3 * (uy * cbz - uz * cby);
-  v1 = uy + v3 * (uz * cbx - ux * cbz);
-  v2 = uz + v3 * (ux * cby - uy * cbx);
-  ux += v4 * (v1 * cbz - v2 * cby);
-  uy += v4 * (v2 * cbx - v0 * cbz);
-  uz += v4 * (v0 * cby - v1 * cbx);
-  ux += hax;
-  uy += hay;
-  uz += haz;
-  a.ux[p] = ux;
-  a.uy[p] = uy;
-  a.uz[p] = uz;
-  // displacement in cell units, push.h:169-182
-  v0 = one / sqrtf_narrow(one + (ux * ux + (uy * uy + uz * uz)));
-  ux *= a.cdt_dx;
-  uy *= a.cdt_dy;
-  uz *= a.cdt_dz;
-  ux *= v0;
-  uy *= v0;
-  uz *= v0;
-  const R mx = dx + ux, my = dy + uy, mz = dz + uz;
-  const R nx = mx + ux, ny = my + uy, nz = mz + uz;
-  const R q = a.w[p] * a.qsp;
-  int moved = 0;
-  if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
-    a.x[p] = nx;
-    a.y[p] = ny;
-    a.z[p] = nz;
-    const R v5 = q * ux * uy * uz * one_third;
-    deposit_streak(a.acc, ii, q, ux, uy, uz, mx, my, mz, v5);
-  } else {
-    R pos[3] = {dx, dy, dz};
-    R rem[3] = {ux, uy, uz};
-    move_p(a, p, pos, ii, q, rem);
-    a.x[p] = pos[0];
-    a.y[p] = pos[1];
-    a.z[p] = pos[2];
-    a.cell[p] = ii;
-    moved = 1;
-  }
-  // one counter update per wave
-  const unsigned long long active = __ballot(1);
-  const unsigned long long movers = __ballot(moved);
-  if ((int)(threadIdx.x & 63) == __ffsll((long long)active) - 1 && movers)
-    atomicAdd(a.movers, (unsigned int)__popcll(movers));
-}
-
-// ---------------------------------------------------------------------------
 // unload_accumulator_array (src/accumulator.cpp:45-122) over [1,n+1]^3.
+// The device accumulators are slot-major, acc[k * Ng + voxel], k = 0..11
+// for jx[0..3], jy[0..3], jz[0..3].
 template <class R>
 __global__ void __launch_bounds__(kBlock)
 k_unload(R* __restrict__ fields, const R* __restrict__ acc, Grid g, R cx, R cy, R cz) {
@@ -354,20 +140,16 @@ k_unload(R* __restrict__ fields, const R* __restrict__ acc, Grid g, R cx, R cy, 
   const int y = 1 + (t / mx) % my;
   const int z = 1 + t / (mx * my);
   const int i = voxel(g, x, y, z);
-  const int dyo = g.sy, dzo = g.sz;
-  const R* a0 = acc + (size_t)i * 12;
-  const R* ay = acc + (size_t)(i - dyo) * 12;
-  const R* az = acc + (size_t)(i - dzo) * 12;
-  const R* ax = acc + (size_t)(i - 1) * 12;
-  const R* ayz = acc + (size_t)(i - dyo - dzo) * 12;
-  const R* azx = acc + (size_t)(i - 1 - dzo) * 12;
-  const R* axy = acc + (size_t)(i - 1 - dyo) * 12;
-  R* jx = fields + 6 * (size_t)g.ncells;
-  R* jy = fields + 7 * (size_t)g.ncells;
-  R* jz = fields + 8 * (size_t)g.ncells;
-  jx[i] = cx * (a0[0] + ay[1] + az[2] + ayz[3]);
-  jy[i] = cy * (a0[4] + az[5] + ax[6] + azx[7]);
-  jz[i] = cz * (a0[8] + ax[9] + ay[10] + axy[11]);
+  const size_t N = g.ncells;
+  const int sy = g.sy, sz = g.sz;
+#define A(k, v) acc[(size_t)(k) * N + (size_t)(v)]
+  R* jx = fields + 6 * N;
+  R* jy = fields + 7 * N;
+  R* jz = fields + 8 * N;
+  jx[i] = cx * (A(0, i) + A(1, i - sy) + A(2, i - sz) + A(3, i - sy - sz));
+  jy[i] = cy * (A(4, i) + A(5, i - sz) + A(6, i - 1) + A(7, i - 1 - sz));
+  jz[i] = cz * (A(8, i) + A(9, i - 1) + A(10, i - sy) + A(11, i - 1 - sy));
+#undef A
 }
 
 // ---------------------------------------------------------------------------
@@ -566,20 +348,6 @@ k_field_energy_partial(const R* __restrict__ fields, Grid g, double* __restrict_
   block_sum_store<2>(v, partials + 2 * blockIdx.x);
 }
 
-// sum w (gamma - 1) over particles -> partials[block][1]
-template <class R>
-__global__ void __launch_bounds__(kBlock)
-k_kinetic_partial(const R* __restrict__ ux, const R* __restrict__ uy, const R* __restrict__ uz,
-                  const R* __restrict__ w, int np, double* __restrict__ partials) {
-  double v[1] = {0};
-  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
-    const double a = ux[p], b = uy[p], c = uz[p];
-    const double u2 = a * a + b * b + c * c;
-    v[0] += (double)w[p] * (u2 / (sqrt(1.0 + u2) + 1.0));
-  }
-  block_sum_store<1>(v, partials + blockIdx.x);
-}
-
 // Final pass: out[q] = scale[q] * sum_b partials[b][q], in block order.
 template <int NQ>
 __global__ void __launch_bounds__(kBlock)
@@ -602,102 +370,6 @@ k_reduce_final(const double* __restrict__ partials, int nblocks, double s0, doub
     out[0] = s0 * res[0];
     if (NQ > 1) out[1] = s1 * res[NQ > 1 ? 1 : 0];
   }
-}
-
-// ---------------------------------------------------------------------------
-// Counting sort by cell: histogram, exclusive scan, scatter.
-
-__global__ void __launch_bounds__(kBlock)
-k_sort_histogram(const int* __restrict__ cell, int np, unsigned int* __restrict__ counts, int ncells,
-                 int* err) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  const int c = cell[p];
-  if ((unsigned)c >= (unsigned)ncells) {
-    atomicOr(err, kErrBadCell);
-    return;
-  }
-  atomicAdd(counts + c, 1u);
-}
-
-// Block-local exclusive scan of 4 * kBlock counts; block totals to sums.
-__global__ void __launch_bounds__(kBlock)
-k_scan_blocks(const unsigned int* __restrict__ in, unsigned int* __restrict__ out, int n,
-              unsigned int* __restrict__ sums) {
-  __shared__ unsigned int s[kBlock];
-  const int base = blockIdx.x * kBlock * 4 + threadIdx.x * 4;
-  unsigned int v[4], tot = 0;
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = (base + k < n) ? in[base + k] : 0u;
-    tot += v[k];
-  }
-  s[threadIdx.x] = tot;
-  __syncthreads();
-  for (int off = 1; off < kBlock; off <<= 1) {
-    unsigned int add = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0u;
-    __syncthreads();
-    s[threadIdx.x] += add;
-    __syncthreads();
-  }
-  unsigned int run = s[threadIdx.x] - tot;  // exclusive prefix of this thread
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    if (base + k < n) out[base + k] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == kBlock - 1) sums[blockIdx.x] = s[kBlock - 1];
-}
-
-// Serial-in-one-block exclusive scan of the block sums (n <= a few 10^4).
-__global__ void __launch_bounds__(kBlock) k_scan_sums(unsigned int* __restrict__ sums, int n) {
-  __shared__ unsigned int s[kBlock];
-  unsigned int carry = 0;
-  for (int base = 0; base < n; base += kBlock) {
-    const int i = base + threadIdx.x;
-    const unsigned int v = (i < n) ? sums[i] : 0u;
-    s[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < kBlock; off <<= 1) {
-      unsigned int add = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0u;
-      __syncthreads();
-      s[threadIdx.x] += add;
-      __syncthreads();
-    }
-    if (i < n) sums[i] = carry + s[threadIdx.x] - v;
-    const unsigned int blocktot = s[kBlock - 1];
-    __syncthreads();
-    carry += blocktot;
-  }
-}
-
-__global__ void __launch_bounds__(kBlock)
-k_scan_add(unsigned int* __restrict__ out, int n, const unsigned int* __restrict__ sums) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] += sums[i / (kBlock * 4)];
-}
-
-template <class R> struct SortArgs {
-  const R* src[7];
-  R* dst[7];
-  const int* src_cell;
-  int* dst_cell;
-  unsigned int* cursor;
-  int np;
-  int ncells;
-};
-
-template <class R>
-__global__ void __launch_bounds__(kBlock) k_sort_scatter(SortArgs<R> a) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.np) return;
-  const int c = a.src_cell[p];
-  if ((unsigned)c >= (unsigned)a.ncells) return;
-  const unsigned int d = atomicAdd(a.cursor + c, 1u);
-  if (d >= (unsigned)a.np) return;
-#pragma unroll
-  for (int k = 0; k < 7; ++k) a.dst[k][d] = a.src[k][p];
-  a.dst_cell[d] = c;
 }
 
 }  // namespace cpic

@@ -54,3 +54,24 @@ def interior_mask(nx, ny, nz):
     m = np.zeros((nz + 2, ny + 2, nx + 2), bool)
     m[1:-1, 1:-1, 1:-1] = True
     return m.ravel()
+
+
+MEMBERS = ("x", "y", "z", "ux", "uy", "uz", "w")
+
+
+def canonical(d):
+    """Particles ordered by (cell, bit patterns of x, y, z, ux, uy, uz, w):
+    the HIP path keeps particles binned by cell, so particle sets are
+    compared in this canonical order."""
+    ui = {4: np.uint32, 8: np.uint64}[np.asarray(d["x"]).dtype.itemsize]
+    keys = [np.ascontiguousarray(d[m]).view(ui) for m in reversed(MEMBERS)] + [np.asarray(d["cell"])]
+    order = np.lexsort(keys)
+    out = {m: np.ascontiguousarray(np.asarray(d[m])[order]) for m in MEMBERS}
+    out["cell"] = np.ascontiguousarray(np.asarray(d["cell"])[order])
+    return out
+
+
+def oracle_particles(st):
+    d = dict(st.p)
+    d["cell"] = st.cell
+    return d
