@@ -36,14 +36,14 @@ def algorithmic_bytes(np_, nc, itemsize):
     return per_p * np_, per_c * nc
 
 
-def run_gpu(deck, steps, warmup, sort_interval, history):
+def run_gpu(deck, steps, warmup, compact_interval, history):
     from cabanapic_amd import Simulation
     parts = deck.load_particles()
     sim = Simulation(deck.nx, deck.ny, deck.nz, deck.num_particles, deck.R)
     sim.upload_particles(*[parts[k] for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
                          parts["cell"])
     del parts
-    sp = deck.step_params(energy_interval=1, sort_interval=sort_interval)
+    sp = deck.step_params(energy_interval=1, compact_interval=compact_interval)
     if warmup:
         sim.step(sp, warmup)
     sim.set_timing(True)
@@ -85,13 +85,28 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
     ap.add_argument("--nppc", type=int, default=32)
-    ap.add_argument("--sort-interval", type=int, default=0)
+    ap.add_argument("--compact-interval", type=int, default=0,
+                    help="steps between column compactions (0 = library default)")
     ap.add_argument("--fp64-steps", type=int, default=20,
                     help="steps of the FP64 path for the FP32/FP64 ratio (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=2,
                     help="steps of the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (torch), the known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
+
+    if args.copy_calibration:
+        import torch
+        n = 1 << 28  # 2^28 float32 = 1 GiB
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": 4 * n, "copy_bytes_written": 4 * n, "launches": 3}))
+        return
 
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
@@ -105,7 +120,7 @@ def main():
     print(f"# workload {d32.name}: {np_} particles, {nc} cells, dt={float(d32.dt):.6g}",
           file=sys.stderr, flush=True)
 
-    t32, h32, st32, err32 = run_gpu(d32, args.steps, args.warmup, args.sort_interval, True)
+    t32, h32, st32, err32 = run_gpu(d32, args.steps, args.warmup, args.compact_interval, True)
     if err32:
         raise SystemExit(f"device error flags {err32}")
     value = np_ * args.steps / t32
@@ -127,7 +142,7 @@ def main():
     if args.fp64_steps > 0:
         d64 = decks.weibel_3d(np.float64, n=args.n, nppc=args.nppc)
         nwin = min(args.fp64_steps, args.steps)
-        t64, h64, st64, err64 = run_gpu(d64, args.fp64_steps, args.warmup, args.sort_interval,
+        t64, h64, st64, err64 = run_gpu(d64, args.fp64_steps, args.warmup, args.compact_interval,
                                         True)
         v64 = np_ * args.fp64_steps / t64
         print(f"# fp64: {1e3 * t64 / args.fp64_steps:.3f} ms/step, {v64:.4g} particle-steps/s",
@@ -165,7 +180,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": d32.name, "cells": [d32.nx, d32.ny, d32.nz], "nppc": d32.nppc,
-                   "particles": np_, "sort_interval": args.sort_interval,
+                   "particles": np_, "compact_interval": args.compact_interval,
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(push_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(push_gbs / HBM_PEAK_GBS, 4),

@@ -40,7 +40,7 @@ class StepParams(ctypes.Structure):
         "qdt_2mc", "cdt_dx", "cdt_dy", "cdt_dz", "qsp", "dx", "dy", "dz", "dt",
         "px", "py", "pz", "dt_eps0", "me", "c")] + [
         ("boundary", ctypes.c_int32), ("energy_interval", ctypes.c_int32),
-        ("sort_interval", ctypes.c_int32), ("kinetic_interval", ctypes.c_int32)]
+        ("compact_interval", ctypes.c_int32), ("kinetic_interval", ctypes.c_int32)]
 
 
 class PlasmaSpec(ctypes.Structure):

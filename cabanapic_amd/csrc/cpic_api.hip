@@ -60,8 +60,10 @@ struct cpic_context {
 
   // particle columns (double-buffered), see cpic_push_kernels.h
   int D = 0;               // rows per column
-  void* cols[2][7] = {};   // x,y,z,ux,uy,uz,w
-  int* cnt[2] = {};        // particles per interior cell
+  void* cols[2] = {};      // kSlotLen reals per slot: x,y,z,ux,uy,uz,w,pad
+  int* cnt[2] = {};        // rows in use per interior cell
+  unsigned long long* mask[2] = {};  // occupied rows, [MW][nc_pad]
+  int MW = 0;              // mask words per cell
   int cur = 0;             // buffer holding the current particles
   // crossing records: list 1 (first and third crossings), list 2 (second)
   int M1 = 0, M2 = 0;
@@ -201,13 +203,15 @@ int stage_clear(cpic_context* c) {
 template <class R>
 ColState<R> col_state(cpic_context* c, int from) {
   ColState<R> cs;
-  for (int k = 0; k < 7; ++k) {
-    cs.in[k] = (const R*)c->cols[from][k];
-    cs.out[k] = (R*)c->cols[1 - from][k];
-  }
+  cs.in = (const R*)c->cols[from];
+  cs.out = (R*)c->cols[1 - from];
   cs.cnt_in = c->cnt[from];
   cs.cnt_out = c->cnt[1 - from];
+  cs.mask_in = c->mask[from];
+  cs.mask_out = c->mask[1 - from];
   cs.D = c->D;
+  cs.MW = c->MW;
+  cs.nc_pad = (int)c->nc_pad;
   return cs;
 }
 
@@ -215,7 +219,8 @@ ColState<R> col_state(cpic_context* c, int from) {
 // accumulate = 1 adds to the accumulators as the reference's push does;
 // accumulate = 0 overwrites them (valid right after a clear).
 template <class R>
-int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, int accumulate) {
+int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, int accumulate,
+               int compact) {
   StageTimer t(c, CPIC_STAGE_PUSH);
   int s;
   if ((s = fill_zero<unsigned int>(c, c->movers, 1))) return s;
@@ -242,6 +247,7 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   p.cdt_dz = cdt_dz;
   p.qsp = qsp;
   p.accumulate = accumulate;
+  p.compact = compact;
   p.err = c->err;
   p.movers = c->movers;
   p.max_count = c->max_count;
@@ -374,12 +380,12 @@ int ensure_columns(cpic_context* c, int D, int M1, int M2) {
   if (c->D == D && c->M1 == M1 && c->M2 == M2 && c->cnt[0]) return CPIC_OK;
   HIP_TRY(hipStreamSynchronize(c->stream));
   for (int b = 0; b < 2; ++b) {
-    for (int k = 0; k < 7; ++k) {
-      if (c->cols[b][k]) HIP_TRY(hipFree(c->cols[b][k]));
-      c->cols[b][k] = nullptr;
-    }
+    if (c->cols[b]) HIP_TRY(hipFree(c->cols[b]));
+    c->cols[b] = nullptr;
     if (c->cnt[b]) HIP_TRY(hipFree(c->cnt[b]));
     c->cnt[b] = nullptr;
+    if (c->mask[b]) HIP_TRY(hipFree(c->mask[b]));
+    c->mask[b] = nullptr;
   }
   void** recs[] = {&c->rec1, &c->rec2, (void**)&c->rc1, (void**)&c->rc2, &c->ovf_rec,
                    (void**)&c->ovf_cell};
@@ -391,9 +397,10 @@ int ensure_columns(cpic_context* c, int D, int M1, int M2) {
   const size_t len = (size_t)c->nc_pad * (size_t)D;
   int s;
   for (int b = 0; b < 2; ++b) {
-    for (int k = 0; k < 7; ++k)
-      if ((s = alloc(&c->cols[b][k], c->rsz() * len))) return s;
+    if ((s = alloc(&c->cols[b], c->rsz() * kSlotLen * len))) return s;
     if ((s = alloc((void**)&c->cnt[b], sizeof(int) * c->nc_pad))) return s;
+    if ((s = alloc((void**)&c->mask[b], sizeof(unsigned long long) * c->nc_pad * ((D + 63) / 64))))
+      return s;
   }
   const size_t per = (size_t)kRecordLen * c->rsz() * 6 * (size_t)c->nc;
   if ((s = alloc(&c->rec1, per * M1))) return s;
@@ -405,7 +412,11 @@ int ensure_columns(cpic_context* c, int D, int M1, int M2) {
   if ((s = alloc((void**)&c->ovf_cell, sizeof(int) * c->ovf_cap))) return s;
   if ((s = fill_zero<int>(c, c->cnt[0], c->nc_pad))) return s;
   if ((s = fill_zero<int>(c, c->cnt[1], c->nc_pad))) return s;
+  const int MW = (D + 63) / 64;
+  if ((s = fill_zero<unsigned long long>(c, c->mask[0], (size_t)MW * c->nc_pad))) return s;
+  if ((s = fill_zero<unsigned long long>(c, c->mask[1], (size_t)MW * c->nc_pad))) return s;
   c->D = D;
+  c->MW = MW;
   c->M1 = M1;
   c->M2 = M2;
   return CPIC_OK;
@@ -518,9 +529,9 @@ int cpic_destroy(cpic_context* c) {
   }
   for (auto e : c->free_events) (void)hipEventDestroy(e);
   for (int b = 0; b < 2; ++b) {
-    for (int k = 0; k < 7; ++k)
-      if (c->cols[b][k]) (void)hipFree(c->cols[b][k]);
+    if (c->cols[b]) (void)hipFree(c->cols[b]);
     if (c->cnt[b]) (void)hipFree(c->cnt[b]);
+    if (c->mask[b]) (void)hipFree(c->mask[b]);
   }
   void* ptrs[] = {c->rec1,   c->rec2,     c->rc1,      c->rc2,     c->ovf_rec,
                   c->ovf_cell, c->ovf_count, c->max_count, c->fields, c->interp,
@@ -579,18 +590,25 @@ int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void*
   if (int s = ensure_columns(c, D, M1, M2)) return s;
   // bin on the host, keeping the input order within each cell
   const size_t rs = c->rsz(), len = c->col_len();
-  std::vector<unsigned char> buf(rs * len * 7, 0);
+  std::vector<unsigned char> buf(rs * len * kSlotLen, 0);
   std::vector<int> fill(c->nc_pad, 0);
   for (int64_t p = 0; p < n; ++p) {
     const int cc = ci[p];
     const size_t slot = ((size_t)(cc >> 6) * D + (size_t)fill[cc]++) * 64 + (cc & 63);
     for (int k = 0; k < 7; ++k)
-      memcpy(&buf[(k * len + slot) * rs], (const unsigned char*)src[k] + p * rs, rs);
+      memcpy(&buf[(slot * kSlotLen + k) * rs], (const unsigned char*)src[k] + p * rs, rs);
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
-  for (int k = 0; k < 7; ++k)
-    HIP_TRY(hipMemcpy(c->cols[0][k], &buf[k * len * rs], rs * len, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->cols[0], buf.data(), buf.size(), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->cnt[0], count.data(), sizeof(int) * c->nc_pad, hipMemcpyHostToDevice));
+  std::vector<unsigned long long> mask((size_t)c->MW * c->nc_pad, 0ull);
+  for (long i = 0; i < c->nc; ++i)
+    for (int wd = 0; wd < c->MW; ++wd) {
+      const int bits = count[i] - 64 * wd;
+      mask[(size_t)wd * c->nc_pad + i] = bits >= 64 ? ~0ull : (bits <= 0 ? 0ull : ((1ull << bits) - 1));
+    }
+  HIP_TRY(hipMemcpy(c->mask[0], mask.data(), sizeof(unsigned long long) * mask.size(),
+                    hipMemcpyHostToDevice));
   c->cur = 0;
   c->np = n;
   return CPIC_OK;
@@ -605,33 +623,38 @@ int cpic_download_particles(cpic_context* c, int64_t n, void* x, void* y, void* 
   void* dst[7] = {x, y, z, ux, uy, uz, w};
   const size_t rs = c->rsz(), len = c->col_len();
   std::vector<int> count(c->nc_pad);
-  std::vector<unsigned char> buf(rs * len);
+  std::vector<unsigned long long> mask((size_t)c->MW * c->nc_pad);
+  std::vector<unsigned char> buf(rs * len * kSlotLen);
   HIP_TRY(hipStreamSynchronize(c->stream));
   HIP_TRY(hipMemcpy(count.data(), c->cnt[c->cur], sizeof(int) * c->nc_pad, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(mask.data(), c->mask[c->cur], sizeof(unsigned long long) * mask.size(),
+                    hipMemcpyDeviceToHost));
+  auto occupied = [&](long i, int r) {
+    return (mask[(size_t)(r >> 6) * c->nc_pad + i] >> (r & 63)) & 1ull;
+  };
   long total = 0;
-  for (long i = 0; i < c->nc; ++i) total += count[i];
+  for (long i = 0; i < c->nc; ++i)
+    for (int r = 0; r < count[i]; ++r) total += occupied(i, r) ? 1 : 0;
   if (total != n)
     return fail(CPIC_ERR_DEVICE, "columns hold %ld particles, %lld expected", total, (long long)n);
   const long X = c->nx + 2, Y = c->ny + 2;
+  HIP_TRY(hipMemcpy(buf.data(), c->cols[c->cur], buf.size(), hipMemcpyDeviceToHost));
   for (int k = -1; k < 7; ++k) {
-    if (k >= 0) {
-      if (!dst[k]) continue;
-      HIP_TRY(hipMemcpy(buf.data(), c->cols[c->cur][k], rs * len, hipMemcpyDeviceToHost));
-    } else if (!cell) {
-      continue;
-    }
-    // cells in interior order (x fastest), rows in column order
+    if (k >= 0 ? !dst[k] : !cell) continue;
+    // cells in interior order (x fastest), occupied rows in row order
     long o = 0;
     for (long i = 0; i < c->nc; ++i) {
       const long ix = i % c->nx, iy = (i / c->nx) % c->ny, iz = i / (c->nx * c->ny);
       const int v = (int)((ix + 1) + X * ((iy + 1) + Y * (iz + 1)));
-      for (int r = 0; r < count[i]; ++r, ++o) {
+      for (int r = 0; r < count[i]; ++r) {
+        if (!occupied(i, r)) continue;
         if (k < 0) {
           cell[o] = v;
         } else {
           const size_t slot = ((size_t)(i >> 6) * c->D + (size_t)r) * 64 + (i & 63);
-          memcpy((unsigned char*)dst[k] + o * rs, &buf[slot * rs], rs);
+          memcpy((unsigned char*)dst[k] + o * rs, &buf[(slot * kSlotLen + k) * rs], rs);
         }
+        ++o;
       }
     }
   }
@@ -735,7 +758,7 @@ int cpic_push(cpic_context* c, double qdt_2mc, double cdt_dx, double cdt_dy, dou
   if (int s = check_periodic(boundary)) return s;
   return with_real(c, [&](auto r) {
     using R = decltype(r);
-    return stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1);
+    return stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1, 1);
   });
 }
 
@@ -829,8 +852,10 @@ int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, doubl
       // the push overwrites every interior accumulator, so the clear is only
       // needed while ghost accumulators may hold data
       if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
+      const int ci = sp->compact_interval > 0 ? sp->compact_interval : 8;
+      const int compact = ((step - 1) % ci == 0) ? 1 : 0;
       if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
-                             (R)sp->qsp, 0)))
+                             (R)sp->qsp, 0, compact)))
         return t;
       if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;
       if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;

@@ -5,11 +5,22 @@
 // cells form a group, and row r of a group is 64 consecutive slots, one per
 // cell:
 //     slot(ci, r) = ((ci / 64) * D + r) * 64 + ci % 64
+// A slot holds one particle as 8 reals (x, y, z, ux, uy, uz, w, unused), two
+// 4-wide vector accesses, so a scattered particle store touches one 32-byte
+// (float) or 64-byte (double) piece of memory.
 // A wave owns the 64 cells of one group, one cell per lane.  Looping over the
 // rows, every load and store of a row is one contiguous 64-wide access, and
 // each lane only ever deposits current into its own cell: the 12 accumulator
 // values of the cell live in the lane's registers and are written once, with
 // no atomics and in a fixed order (so the deposit is deterministic).
+//
+// Occupancy.  A column uses rows [0, len) and a bit mask per cell marks the
+// occupied rows (words mask[w * Nc_pad + ci], bit r of word w = row 64w + r).
+// The push normally keeps a particle that stays in its cell in its row, so
+// the stores of a row stay one contiguous access; the rows of particles that
+// left become holes, which the particles arriving from the neighbours fill
+// before the column grows.  Every few steps a compacting push packs each
+// column again (stayers to rows 0, 1, 2, ...).
 //
 // Cell crossings.  The reference's move_p loop (src/move_p.h:93-371) walks a
 // particle through up to four streaks, each in one cell.  Here every streak
@@ -19,7 +30,9 @@
 //            to the output column ("stayer").  A particle that crosses face
 //            d is written as a record (position on the entry face of the
 //            neighbour, remaining displacement, momentum, weight) into slot
-//            k of the (cell, d) record list.
+//            k of the (cell, d) record list.  The list of one (cell, d) is
+//            contiguous, so each lane writes and later reads its records
+//            sequentially.
 //   pass 2-4 the owner of the neighbour cell reads the records addressed to
 //            it from its six neighbours, runs the next streak, and either
 //            appends the particle to its own column or forwards a new record.
@@ -34,6 +47,7 @@
 namespace cpic {
 
 constexpr int kRecordLen = 12;  // pos[3], rem[3], u[3], w, 2 pad
+constexpr int kSlotLen = 8;     // x, y, z, ux, uy, uz, w, pad
 
 __device__ __forceinline__ size_t col_slot(int ci, int row, int D) {
   return ((size_t)(ci >> 6) * (size_t)D + (size_t)row) * 64 + (size_t)(ci & 63);
@@ -169,18 +183,85 @@ __device__ __forceinline__ int streak(R pos[3], R rem[3], R q, R acc[12]) {
 }
 
 template <class R> struct ColState {
-  const R* in[7];  // x,y,z,ux,uy,uz,w input columns
-  R* out[7];       // output columns
-  const int* cnt_in;
+  const R* in;  // input columns, kSlotLen reals per slot
+  R* out;       // output columns
+  const int* cnt_in;   // rows in use per cell (len)
   int* cnt_out;
+  const unsigned long long* mask_in;  // occupied rows, [w][Nc_pad]
+  unsigned long long* mask_out;
   int D;
+  int MW;      // mask words per cell = ceil(D / 64)
+  int nc_pad;  // interior cells rounded up to whole groups
+};
+
+__device__ __forceinline__ unsigned long long low_bits(int n) {
+  return n >= 64 ? ~0ull : (n <= 0 ? 0ull : ((1ull << n) - 1));
+}
+
+// Hands out free rows of one lane's output column: first the holes below
+// len (in row order), then new rows at the end.  The mask word being edited
+// is cached in a register and written back when the cursor leaves it.
+template <class R> struct RowCursor {
+  int len, hw;
+  unsigned long long word, freeb;
+  bool have, append;
+  __device__ void start(int l) {
+    len = l;
+    hw = -1;
+    word = freeb = 0;
+    have = append = false;
+  }
+  __device__ int next(const ColState<R>& cs, int ci) {
+    for (int guard = 0; guard < 64; ++guard) {
+      if (!append) {
+        if (have && freeb) {
+          const int b = __ffsll((long long)freeb) - 1;
+          freeb &= freeb - 1;
+          word |= 1ull << b;
+          return hw * 64 + b;
+        }
+        if (have) cs.mask_out[(size_t)hw * cs.nc_pad + ci] = word;
+        have = false;
+        ++hw;
+        if (hw * 64 >= len) {
+          append = true;
+          continue;
+        }
+        word = cs.mask_out[(size_t)hw * cs.nc_pad + ci];
+        freeb = ~word & low_bits(len - hw * 64);
+        have = true;
+      } else {
+        if (len >= cs.D) return -1;
+        const int row = len++;
+        const int w = row >> 6;
+        if (!have || w != hw) {
+          if (have) cs.mask_out[(size_t)hw * cs.nc_pad + ci] = word;
+          hw = w;
+          word = cs.mask_out[(size_t)hw * cs.nc_pad + ci];
+          have = true;
+        }
+        word |= 1ull << (row & 63);
+        return row;
+      }
+    }
+    return -1;
+  }
+  __device__ void finish(const ColState<R>& cs, int ci) {
+    if (have) cs.mask_out[(size_t)hw * cs.nc_pad + ci] = word;
+    have = false;
+  }
 };
 
 template <class R> struct RecordList {
-  R* rec;    // [6][M][Nc] records of kRecordLen
+  R* rec;    // [Nc][6][M] records of kRecordLen
   int* cnt;  // [6][Nc]
   int M;
 };
+
+template <class R>
+__device__ __forceinline__ R* record_at(const RecordList<R>& L, int ci, int d, int k) {
+  return L.rec + ((((size_t)ci * 6 + d) * L.M) + k) * kRecordLen;
+}
 
 template <class R> struct Overflow {
   R* rec;           // [cap] records
@@ -200,7 +281,7 @@ __device__ __forceinline__ void put_record(const RecordList<R>& L, const Overflo
   c.x = uz;     c.y = w;      c.z = 0;      c.w = 0;
   V4* dst;
   if (k < L.M) {
-    dst = reinterpret_cast<V4*>(L.rec + (((size_t)d * L.M + k) * g.nc + ci) * kRecordLen);
+    dst = reinterpret_cast<V4*>(record_at(L, ci, d, k));
   } else {
     const unsigned slot = atomicAdd(ovf.count, 1u);
     if (slot >= (unsigned)ovf.cap) {
@@ -220,14 +301,13 @@ __device__ __forceinline__ void put_record(const RecordList<R>& L, const Overflo
 template <class R>
 __device__ __forceinline__ void store_particle(const ColState<R>& cs, int ci, int row, R x, R y, R z,
                                                R ux, R uy, R uz, R w) {
-  const size_t s = col_slot(ci, row, cs.D);
-  cs.out[0][s] = x;
-  cs.out[1][s] = y;
-  cs.out[2][s] = z;
-  cs.out[3][s] = ux;
-  cs.out[4][s] = uy;
-  cs.out[5][s] = uz;
-  cs.out[6][s] = w;
+  typedef typename Vec4T<R>::type V4;
+  V4 a, b;
+  a.x = x;  a.y = y;  a.z = z;  a.w = ux;
+  b.x = uy; b.y = uz; b.z = w;  b.w = 0;
+  V4* dst = reinterpret_cast<V4*>(cs.out + col_slot(ci, row, cs.D) * kSlotLen);
+  dst[0] = a;
+  dst[1] = b;
 }
 
 template <class R> struct PushPass {
@@ -240,6 +320,7 @@ template <class R> struct PushPass {
   int ncells;  // Ng
   R qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
   int accumulate;  // 1: acc += (push semantics), 0: acc = (after a clear)
+  int compact;     // 1: pack stayers to rows 0,1,..; 0: keep stayers in their row
   int* err;
   unsigned* movers;
   int* max_count;
@@ -250,8 +331,8 @@ template <class R>
 __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = ci < a.g.nc;
-  const int n = valid ? a.cs.cnt_in[ci] : 0;
-  const int nmax = wave_max_int(n);
+  const int len = valid ? a.cs.cnt_in[ci] : 0;
+  const int lmax = wave_max_int(len);
   const int v = valid ? interior_to_voxel(a.g, ci) : 0;
   typedef typename Vec4T<R>::type V4;
   V4 e0 = {}, e1 = {}, e2 = {}, b0 = {}, b1 = {};
@@ -270,15 +351,20 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   R acc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0;
-  int ns = 0, nmov = 0;
+  int ns = 0, hi = 0, nmov = 0;
   int kd[6] = {0, 0, 0, 0, 0, 0};
+  unsigned long long m_in = 0, m_out = 0;
+  const size_t MP = a.cs.nc_pad;
 
-  for (int j = 0; j < nmax; ++j) {
-    if (j < n) {
-      const size_t s = col_slot(ci, j, a.cs.D);
-      const R dx = a.cs.in[0][s], dy = a.cs.in[1][s], dz = a.cs.in[2][s];
-      R ux = a.cs.in[3][s], uy = a.cs.in[4][s], uz = a.cs.in[5][s];
-      const R w = a.cs.in[6][s];
+  for (int j = 0; j < lmax; ++j) {
+    const int b = j & 63;
+    if (b == 0) m_in = valid ? a.cs.mask_in[(size_t)(j >> 6) * MP + ci] : 0ull;
+    if (j < len && ((m_in >> b) & 1ull)) {
+      const V4* src = reinterpret_cast<const V4*>(a.cs.in + col_slot(ci, j, a.cs.D) * kSlotLen);
+      const V4 pa = src[0], pb = src[1];
+      const R dx = pa.x, dy = pa.y, dz = pa.z;
+      R ux = pa.w, uy = pb.x, uz = pb.y;
+      const R w = pb.z;
       // field gather, push.h:124-138
       const R hax = qdt_2mc * ((e0.x + dy * e0.y) + dz * (e0.z + dy * e0.w));
       const R hay = qdt_2mc * ((e1.x + dz * e1.y) + dx * (e1.z + dz * e1.w));
@@ -316,26 +402,48 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
       const R mx = dx + ddx, my = dy + ddy, mz = dz + ddz;
       const R nx = mx + ddx, ny = my + ddy, nz = mz + ddz;
       const R q = w * a.qsp;
+      R px, py, pz;
+      bool stays;
       if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
         const R v5 = q * ddx * ddy * ddz * one_third;
         R o[12];
         streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
 #pragma unroll
         for (int k = 0; k < 12; ++k) acc[k] += o[k];
-        store_particle(a.cs, ci, ns++, nx, ny, nz, ux, uy, uz, w);
+        px = nx;
+        py = ny;
+        pz = nz;
+        stays = true;
       } else {
         ++nmov;
         R pos[3] = {dx, dy, dz};
         R rem[3] = {ddx, ddy, ddz};
         const int d = streak(pos, rem, q, acc);
-        if (d < 0) {
-          store_particle(a.cs, ci, ns++, pos[0], pos[1], pos[2], ux, uy, uz, w);
-        } else {
+        px = pos[0];
+        py = pos[1];
+        pz = pos[2];
+        stays = d < 0;
+        if (!stays) {
           int k = kd[d];
           put_record(a.out, a.ovf, a.g, ci, d, k, pos, rem, ux, uy, uz, w, a.err);
           kd[d] = k;
         }
       }
+      if (stays) {
+        int row;
+        if (a.compact) {
+          row = ns++;
+        } else {
+          row = j;
+          m_out |= 1ull << b;
+          hi = j + 1;
+        }
+        store_particle(a.cs, ci, row, px, py, pz, ux, uy, uz, w);
+      }
+    }
+    if (!a.compact && (b == 63 || j == lmax - 1)) {
+      if (valid) a.cs.mask_out[(size_t)(j >> 6) * MP + ci] = m_out;
+      m_out = 0;
     }
   }
   if (valid) {
@@ -347,14 +455,20 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
 #pragma unroll
       for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc[k];
     }
-    a.cs.cnt_out[ci] = ns;
+    if (a.compact) {
+      for (int w = 0; w < a.cs.MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = low_bits(ns - 64 * w);
+      a.cs.cnt_out[ci] = ns;
+    } else {
+      for (int w = (lmax + 63) >> 6; w < a.cs.MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
+      a.cs.cnt_out[ci] = hi;
+    }
 #pragma unroll
     for (int d = 0; d < 6; ++d) a.out.cnt[(size_t)d * a.g.nc + ci] = min(kd[d], a.out.M);
   }
   const int wm = wave_sum_int(nmov);
   if ((threadIdx.x & 63) == 0) {
     if (wm) atomicAdd(a.movers, (unsigned)wm);
-    atomicMax(a.max_count, nmax);
+    atomicMax(a.max_count, lmax);
   }
 }
 
@@ -371,41 +485,69 @@ template <class R> struct FinishPass {
   int* err;
 };
 
+__device__ __forceinline__ int sel6(int d, int a0, int a1, int a2, int a3, int a4, int a5) {
+  return d == 0 ? a0 : d == 1 ? a1 : d == 2 ? a2 : d == 3 ? a3 : d == 4 ? a4 : a5;
+}
+
 // Passes 2-4: the next streak of every particle that entered the lane's cell.
+// Each lane walks its own arrivals (all directions, in direction then record
+// order) in one loop, so a wave iterates max over lanes of the lane totals.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = ci < a.g.nc;
-  const int v = valid ? interior_to_voxel(a.g, ci) : 0;
-  int na = valid ? a.cs.cnt_out[ci] : 0;
-  R acc[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = 0;
-  bool any = false;
+  const size_t NC = a.g.nc;
+  int s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+  int c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+  if (valid) {
+    // the source of a record of direction d sits across the opposite face
+    s0 = face_neighbor(a.g, ci, 3);
+    s1 = face_neighbor(a.g, ci, 4);
+    s2 = face_neighbor(a.g, ci, 5);
+    s3 = face_neighbor(a.g, ci, 0);
+    s4 = face_neighbor(a.g, ci, 1);
+    s5 = face_neighbor(a.g, ci, 2);
+    c0 = a.in.cnt[0 * NC + s0];
+    c1 = a.in.cnt[1 * NC + s1];
+    c2 = a.in.cnt[2 * NC + s2];
+    c3 = a.in.cnt[3 * NC + s3];
+    c4 = a.in.cnt[4 * NC + s4];
+    c5 = a.in.cnt[5 * NC + s5];
+  }
+  const int total = c0 + c1 + c2 + c3 + c4 + c5;
+  const int tmax = wave_max_int(total);
   int kd[6] = {0, 0, 0, 0, 0, 0};
-  typedef typename Vec4T<R>::type V4;
-  for (int d = 0; d < 6; ++d) {
-    const int si = valid ? face_neighbor(a.g, ci, (d + 3) % 6) : 0;
-    const int c = valid ? a.in.cnt[(size_t)d * a.g.nc + si] : 0;
-    const int cmax = wave_max_int(c);
-    for (int k = 0; k < cmax; ++k) {
-      if (k < c) {
-        const V4* src =
-            reinterpret_cast<const V4*>(a.in.rec + (((size_t)d * a.in.M + k) * a.g.nc + si) * kRecordLen);
+  if (tmax > 0) {
+    const int v = valid ? interior_to_voxel(a.g, ci) : 0;
+    RowCursor<R> rows;
+    rows.start(valid ? a.cs.cnt_out[ci] : 0);
+    R acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = 0;
+    typedef typename Vec4T<R>::type V4;
+    int d = 0, k = 0;
+    for (int i = 0; i < tmax; ++i) {
+      if (i < total) {
+        while (d < 6 && k >= sel6(d, c0, c1, c2, c3, c4, c5)) {
+          k = 0;
+          ++d;
+        }
+        if (d >= 6) break;  // unreachable while the counts are consistent
+        const int si = sel6(d, s0, s1, s2, s3, s4, s5);
+        const V4* src = reinterpret_cast<const V4*>(record_at(a.in, si, d, k));
+        ++k;
         const V4 r0 = src[0], r1 = src[1], r2 = src[2];
         R pos[3] = {r0.x, r0.y, r0.z};
         R rem[3] = {r0.w, r1.x, r1.y};
         const R ux = r1.z, uy = r1.w, uz = r2.x, w = r2.y;
         const R q = w * a.qsp;
-        any = true;
         const int d2 = streak(pos, rem, q, acc);
         if (d2 < 0) {
-          if (na < a.cs.D) {
-            store_particle(a.cs, ci, na, pos[0], pos[1], pos[2], ux, uy, uz, w);
-            ++na;
-          } else {
+          const int row = rows.next(a.cs, ci);
+          if (row >= 0)
+            store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w);
+          else
             atomicOr(a.err, kErrColumnFull);
-          }
         } else {
           int kk = a.last ? a.out.M : kd[d2];  // last pass: straight to overflow
           put_record(a.out, a.ovf, a.g, ci, d2, kk, pos, rem, ux, uy, uz, w, a.err);
@@ -413,18 +555,19 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
         }
       }
     }
+    if (valid) {
+      rows.finish(a.cs, ci);
+      a.cs.cnt_out[ci] = rows.len;
+      if (total > 0) {
+        const size_t N = a.ncells;
+#pragma unroll
+        for (int kq = 0; kq < 12; ++kq) a.acc[kq * N + v] += acc[kq];
+      }
+    }
   }
-  if (valid) {
-    a.cs.cnt_out[ci] = na;
-    if (!a.last) {
+  if (valid && !a.last) {
 #pragma unroll
-      for (int d = 0; d < 6; ++d) a.out.cnt[(size_t)d * a.g.nc + ci] = min(kd[d], a.out.M);
-    }
-    if (any) {
-      const size_t N = a.ncells;
-#pragma unroll
-      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc[k];
-    }
+    for (int dd = 0; dd < 6; ++dd) a.out.cnt[(size_t)dd * NC + ci] = min(kd[dd], a.out.M);
   }
 }
 
@@ -478,10 +621,12 @@ __global__ void __launch_bounds__(kBlock) k_spill_pass(SpillPass<R> a) {
       continue;
     }
     const int row = atomicAdd(a.cs.cnt_out + ci, 1);
-    if (row < a.cs.D)
+    if (row < a.cs.D) {
       store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w);
-    else
+      atomicOr(a.cs.mask_out + (size_t)(row >> 6) * a.cs.nc_pad + ci, 1ull << (row & 63));
+    } else {
       atomicOr(a.err, kErrColumnFull);
+    }
   }
 }
 
@@ -492,11 +637,14 @@ k_kinetic_columns(ColState<R> cs, int nc, double* __restrict__ partials) {
   double v[1] = {0};
   for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nc; ci += gridDim.x * blockDim.x) {
     const int n = cs.cnt_in[ci];
+    unsigned long long m = 0;
     for (int j = 0; j < n; ++j) {
-      const size_t s = col_slot(ci, j, cs.D);
-      const double a = cs.in[3][s], b = cs.in[4][s], c = cs.in[5][s];
+      if ((j & 63) == 0) m = cs.mask_in[(size_t)(j >> 6) * cs.nc_pad + ci];
+      if (!((m >> (j & 63)) & 1ull)) continue;
+      const R* p = cs.in + col_slot(ci, j, cs.D) * kSlotLen;
+      const double a = p[3], b = p[4], c = p[5];
       const double u2 = a * a + b * b + c * c;
-      v[0] += (double)cs.in[6][s] * (u2 / (sqrt(1.0 + u2) + 1.0));
+      v[0] += (double)p[6] * (u2 / (sqrt(1.0 + u2) + 1.0));
     }
   }
   block_sum_store<1>(v, partials + blockIdx.x);

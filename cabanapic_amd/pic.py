@@ -217,11 +217,11 @@ class Simulation:
 
 def make_step_params(qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, dx, dy, dz, dt, px, py, pz,
                      dt_eps0, me=1.0, c=1.0, boundary=BOUNDARY_PERIODIC, energy_interval=1,
-                     sort_interval=0, kinetic_interval=0):
+                     compact_interval=0, kinetic_interval=0):
     return StepParams(float(qdt_2mc), float(cdt_dx), float(cdt_dy), float(cdt_dz), float(qsp),
                       float(dx), float(dy), float(dz), float(dt), float(px), float(py),
                       float(pz), float(dt_eps0), float(me), float(c), int(boundary),
-                      int(energy_interval), int(sort_interval), int(kinetic_interval))
+                      int(energy_interval), int(compact_interval), int(kinetic_interval))
 
 
 __all__ = ["Simulation", "make_step_params", "CpicError", "PARTICLE_MEMBERS", "FIELD_MEMBERS",

@@ -63,8 +63,9 @@ typedef struct cpic_step_params {
   int32_t energy_interval;  /* E,B energies every k steps (0 = never);
                                the reference dumps every step
                                (ENERGY_DUMP_INTERVAL, example.cpp:53-55)   */
-  int32_t sort_interval;    /* cell sort every k steps (0 = never, as in the
-                               reference, example/example.cpp:226-228)      */
+  int32_t compact_interval; /* particles stay binned by cell; every k steps
+                               the push also packs each cell's column
+                               (0 = every 8 steps)                           */
   int32_t kinetic_interval; /* kinetic energy every k steps (0 = never)     */
 } cpic_step_params;
 
