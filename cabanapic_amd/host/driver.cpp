@@ -1,0 +1,229 @@
+// driver.cpp -- the simulation driver of the reference (example/example.cpp)
+// on the MI355X hot path: derives the deck, builds the initial state on the
+// host through the deck's initializers, uploads it, runs the step loop through
+// the C-ABI and writes energies.txt, then calls the deck's finalizer.
+//
+// Usage: cbnpic [--stages] [--chunk N] [--dump-initial FILE]
+//   default   the fused cpic_step, energies of every step kept on the device
+//             and written after each chunk of N steps (default 200)
+//   --stages  one C-ABI call per stage and step, in the reference's order
+//             (example/example.cpp:216-279), energies read back every step
+//   --dump-initial FILE
+//             write the derived constants and the initial particles to FILE
+//             and exit without touching the GPU (checks of the deck layer)
+//   --dump-constants FILE
+//             the same without the particles
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <string>
+#include <vector>
+
+#include "../../include/cabanapic_amd.h"
+#include "deck.h"
+
+Input_Deck deck;
+
+namespace {
+
+void die(const char* what, int code) {
+  std::fprintf(stderr, "cbnpic: %s failed (%d): %s\n", what, code, cpic_last_error());
+  std::exit(1);
+}
+
+#define CPIC_CALL(expr)              \
+  do {                               \
+    const int rc_ = (expr);          \
+    if (rc_ != CPIC_OK) die(#expr, rc_); \
+  } while (0)
+
+// energies.txt in the reference's format (src/fields.h:741-762): the file is
+// truncated by the first step and appended afterwards; each line is also
+// printed as "%d %f %e %e".
+void dump_energy_line(int step, real_t time, real_t e, real_t b) {
+  std::ofstream f;
+  if (step == 1)
+    f.open("energies.txt", std::ofstream::out | std::ofstream::trunc);
+  else
+    f.open("energies.txt", std::ios::app);
+  f << step << " " << time << " " << e << " " << b << std::endl;
+  std::printf("%d %f %e %e\n", step, (double)time, (double)e, (double)b);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  bool stages = false;
+  long chunk = 200;
+  const char* dump = nullptr;
+  bool dump_particles = true;
+  for (int i = 1; i < argc; ++i) {
+    if (!std::strcmp(argv[i], "--stages")) {
+      stages = true;
+    } else if (!std::strcmp(argv[i], "--chunk") && i + 1 < argc) {
+      chunk = std::atol(argv[++i]);
+      if (chunk < 1) chunk = 1;
+    } else if (!std::strcmp(argv[i], "--dump-initial") && i + 1 < argc) {
+      dump = argv[++i];
+    } else if (!std::strcmp(argv[i], "--dump-constants") && i + 1 < argc) {
+      dump = argv[++i];
+      dump_particles = false;
+    } else {
+      std::fprintf(stderr, "usage: %s [--stages] [--chunk N] [--dump-initial|--dump-constants FILE]\n",
+                   argv[0]);
+      return 2;
+    }
+  }
+  std::printf("#Running on the MI355X hot path (real_t = %s)\n",
+              sizeof(real_t) == 4 ? "float" : "double");
+
+  deck.derive_params();
+  deck.print_run_details();
+
+  // driver constants with the reference's types (example/example.cpp:65-114)
+  const int npc = (int)deck.nppc;
+  const int nx = (int)deck.nx, ny = (int)deck.ny, nz = (int)deck.nz;
+  const int num_ghosts = (int)deck.num_ghosts;
+  const size_t num_cells = deck.num_cells;
+  real_t dxp = 2.f / (npc);
+  const real_t dx = deck.dx, dy = deck.dy, dz = deck.dz;
+  real_t dt = deck.dt;
+  real_t c = deck.c;
+  real_t Lx = deck.len_x, Ly = deck.len_y, Lz = deck.len_z;
+  real_t v0 = deck.v0;
+  real_t eps0 = deck.eps;
+  real_t Npe = deck.Npe;
+  size_t Ne = (size_t)deck.Ne;
+  real_t qsp = deck.qsp;
+  real_t me = deck.me;
+  real_t qdt_2mc = qsp * dt / (2 * me * c);
+  real_t cdt_dx = c * dt / dx;
+  real_t cdt_dy = c * dt / dy;
+  real_t cdt_dz = c * dt / dz;
+  real_t dt_eps0 = dt / eps0;
+  real_t frac = 1.0f;
+  real_t we = (real_t)Npe / (real_t)Ne;
+  const size_t num_particles = (size_t)deck.num_particles;
+  std::printf("nppc %d nx %d ny %d nz %d \n", npc, nx, ny, nz);
+  std::printf("ne %zu npe %e we %e qsp %e\n", Ne, (double)Npe, (double)we, (double)qsp);
+  std::printf("c %e dt %e dx %e cdt_dx %e \n", (double)c, (double)dt, (double)dx, (double)cdt_dx);
+
+  particle_list_t particles(dump && !dump_particles ? 0 : num_particles);
+  if (!dump || dump_particles)
+    deck.initialize_particles(particles, nx, ny, nz, num_ghosts, dxp, npc, we, v0);
+  field_array_t fields(num_cells);
+  deck.initialize_fields(fields, nx, ny, nz, num_ghosts, Lx, Ly, Lz, dx, dy, dz);
+  const Boundary boundary = deck.BOUNDARY_TYPE;
+
+  // (example/example.cpp:179-181)
+  const real_t px = (nx > 1) ? frac * c * dt / dx : 0;
+  const real_t py = (ny > 1) ? frac * c * dt / dy : 0;
+  const real_t pz = (nz > 1) ? frac * c * dt / dz : 0;
+  const int num_steps = deck.num_steps;
+  std::printf("#num_step = %d\n#dt*wpe = %f\n#we = %f\n", num_steps, (double)dt, (double)we);
+
+  if (dump) {
+    // int64 np, int32 sizeof(real_t), 16 constants as double, then the
+    // particle members x,y,z,ux,uy,uz,w (real_t) and the int32 cells
+    std::FILE* f = std::fopen(dump, "wb");
+    if (!f) return 1;
+    const int64_t n = dump_particles ? (int64_t)num_particles : 0;
+    const int32_t rs = (int32_t)sizeof(real_t);
+    const double k[16] = {qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, dx,  dy, dz,
+                          dt,      px,     py,     pz,     dt_eps0, dxp, we, (double)num_steps};
+    std::fwrite(&n, sizeof n, 1, f);
+    std::fwrite(&rs, sizeof rs, 1, f);
+    std::fwrite(k, sizeof k, 1, f);
+    for (int m = PositionX; m <= Weight; ++m)
+      std::fwrite(particles.data(m), sizeof(real_t), (size_t)n, f);
+    std::fwrite(particles.cells(), sizeof(int32_t), (size_t)n, f);
+    std::fclose(f);
+    return 0;
+  }
+
+  if (deck.perform_uncenter) {
+    std::fprintf(stderr, "cbnpic: perform_uncenter is not supported by this build\n");
+    return 1;
+  }
+  if (boundary != Boundary::Periodic) {
+    // the reference's field solver exits for anything but periodic
+    // boundaries (src/fields.h:21-25,113-117)
+    std::fprintf(stderr, "cbnpic: only periodic boundaries are supported\n");
+    return 1;
+  }
+
+  cpic_config cfg{};
+  cfg.real_size = (int32_t)sizeof(real_t);
+  cfg.num_ghosts = num_ghosts;
+  cfg.nx = nx;
+  cfg.ny = ny;
+  cfg.nz = nz;
+  cfg.max_particles = (int64_t)num_particles;
+  cpic_context* ctx = nullptr;
+  CPIC_CALL(cpic_create(&cfg, &ctx));
+  CPIC_CALL(cpic_upload_particles(ctx, (int64_t)num_particles, particles.data(PositionX),
+                                  particles.data(PositionY), particles.data(PositionZ),
+                                  particles.data(VelocityX), particles.data(VelocityY),
+                                  particles.data(VelocityZ), particles.data(Weight),
+                                  particles.cells()));
+  CPIC_CALL(cpic_upload_fields(ctx, fields.data()));
+
+  if (stages) {
+    for (int step = 1; step <= num_steps; step++) {
+      CPIC_CALL(cpic_load_interpolator_array(ctx));
+      CPIC_CALL(cpic_clear_accumulator_array(ctx));
+      CPIC_CALL(cpic_push(ctx, qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, CPIC_BOUNDARY_PERIODIC));
+      CPIC_CALL(cpic_unload_accumulator_array(ctx, dx, dy, dz, dt));
+      CPIC_CALL(cpic_advance_b(ctx, real_t(0.5) * px, real_t(0.5) * py, real_t(0.5) * pz));
+      CPIC_CALL(cpic_advance_e(ctx, px, py, pz, dt_eps0));
+      CPIC_CALL(cpic_advance_b(ctx, real_t(0.5) * px, real_t(0.5) * py, real_t(0.5) * pz));
+      double e = 0, b = 0;
+      CPIC_CALL(cpic_energies(ctx, &e, &b));
+      dump_energy_line(step, step * dt, (real_t)e, (real_t)b);
+    }
+  } else {
+    cpic_step_params sp{};
+    sp.qdt_2mc = qdt_2mc;
+    sp.cdt_dx = cdt_dx;
+    sp.cdt_dy = cdt_dy;
+    sp.cdt_dz = cdt_dz;
+    sp.qsp = qsp;
+    sp.dx = dx;
+    sp.dy = dy;
+    sp.dz = dz;
+    sp.dt = dt;
+    sp.px = px;
+    sp.py = py;
+    sp.pz = pz;
+    sp.dt_eps0 = dt_eps0;
+    sp.me = me;
+    sp.c = c;
+    sp.boundary = CPIC_BOUNDARY_PERIODIC;
+    sp.energy_interval = 1;
+    sp.compact_interval = 0;
+    sp.kinetic_interval = 0;
+    std::vector<double> hist;
+    for (int first = 1; first <= num_steps; first += (int)chunk) {
+      const int n = (int)std::min<long>(chunk, num_steps - first + 1);
+      hist.assign(3 * (size_t)n, 0.0);
+      CPIC_CALL(cpic_step(ctx, &sp, n, hist.data()));
+      for (int k = 0; k < n; ++k) {
+        const int step = first + k;
+        dump_energy_line(step, step * dt, (real_t)hist[3 * k], (real_t)hist[3 * k + 1]);
+      }
+    }
+  }
+  int32_t flags = 0;
+  CPIC_CALL(cpic_device_errors(ctx, &flags));
+  CPIC_CALL(cpic_destroy(ctx));
+  if (flags) {
+    std::fprintf(stderr, "cbnpic: device error flags %d\n", flags);
+    return 1;
+  }
+
+  // let the deck check or post-process the run (example/example.cpp:287)
+  deck.finalize();
+  return 0;
+}
