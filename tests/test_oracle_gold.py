@@ -96,3 +96,25 @@ def test_particle_count_and_cells_stay_interior(oracle):
     assert iz.min() >= 1 and iz.max() <= nz
     for k in ("x", "y", "z"):
         assert np.abs(st.p[k]).max() <= 1.0
+
+
+def test_float_e_rule_is_order_sensitive(oracle):
+    """In float the E window of the reference rule lies in the chaotic
+    nonlinear phase: permuting the particles, which changes nothing but the
+    summation order of the deposit, moves the E error across the 10 % line
+    (seed 3: about 14 %, seed 0: about 5 %) while B stays below 1 %.  A float
+    implementation with another deposit order is therefore held to the B rule
+    and to the order-only E envelope, not to the E line itself."""
+    errs = {}
+    for seed in (0, 3):
+        d, st = oracle.setup_2stream_em(np.float32)
+        perm = np.random.default_rng(seed).permutation(st.np)
+        for k in st.p:
+            st.p[k][:] = st.p[k][perm]
+        st.cell[:] = st.cell[perm]
+        h, _ = st.run(d["consts"], d["num_steps"])
+        ok, ee, eb = energy_rule.reference_rule(h[:, 2], h[:, 3], "float")
+        errs[seed] = (ok, ee, eb)
+        assert eb < 0.01, (seed, eb)
+        assert ee < 0.20, (seed, ee)
+    assert errs[0][0] and not errs[3][0], errs
