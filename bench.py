@@ -36,6 +36,22 @@ def algorithmic_bytes(np_, nc, itemsize):
     return per_p * np_, per_c * nc
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "push_stage_weibel128x32_traffic.md")
+
+
+def measured_push_traffic(n, nppc):
+    """HBM bytes per push stage from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (scaled by the copy calibration), for the workload it
+    was measured on; None for any other workload."""
+    if (n, nppc) != (128, 32) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    with open(TRAFFIC_PROFILE) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_push_stage:"):
+                return float(line.split(":")[1])
+    return None
+
+
 def run_gpu(deck, steps, warmup, compact_interval, history):
     from cabanapic_amd import Simulation
     parts = deck.load_particles()
@@ -89,7 +105,7 @@ def main():
                     help="steps between column compactions (0 = library default)")
     ap.add_argument("--fp64-steps", type=int, default=20,
                     help="steps of the FP64 path for the FP32/FP64 ratio (0 = skip)")
-    ap.add_argument("--cpu-steps", type=int, default=2,
+    ap.add_argument("--cpu-steps", type=int, default=30,
                     help="steps of the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -184,7 +200,10 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(push_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(push_gbs / HBM_PEAK_GBS, 4),
-                     "kernel": "k_push", "traffic": None},
+                     "kernel": "push stage: k_push_pass + 3 x k_finish_pass + k_spill_pass",
+                     "traffic": measured_push_traffic(args.n, args.nppc),
+                     "traffic_unit": "bytes per push stage (one step)",
+                     "algorithmic_bytes": bp},
         "cpu_baseline": cpu,
         "extra": extra,
     }

@@ -145,9 +145,10 @@ int cpic_energies(cpic_context *ctx, double *e_energy, double *b_energy);
  * counterpart).  Synchronous. */
 int cpic_kinetic_energy(cpic_context *ctx, double me, double c, double *ke);
 
-/* Counting sort of the particles by cell (the commented-out
- * Cabana::sortByKey of example/example.cpp:226-228).  The order of the
- * particles within one cell is unspecified. */
+/* Sort of the particles by cell (the commented-out Cabana::sortByKey of
+ * example/example.cpp:226-228).  Particles are binned by cell from upload
+ * on and every push keeps them binned, so this only validates the context;
+ * the order of the particles within one cell is unspecified. */
 int cpic_sort_particles(cpic_context *ctx);
 
 /* nsteps iterations of the step loop body, example/example.cpp:216-279:
