@@ -2,12 +2,15 @@
 
 The shared library is built in-tree (cabanapic_amd/libcabanapic_amd.so) by
 ``__graft_entry__.build()``; loading fails loudly when it is missing, so no
-caller can silently fall back to another implementation.
+caller can silently fall back to another implementation.  The environment
+variable CABANAPIC_AMD_LIB names another build of the same library (an
+installed copy, or a variant build being compared).
 """
 import ctypes
 import os
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libcabanapic_amd.so")
+LIB_PATH = os.environ.get("CABANAPIC_AMD_LIB") or os.path.join(
+    os.path.dirname(os.path.abspath(__file__)), "libcabanapic_amd.so")
 
 CPIC_OK = 0
 CPIC_ERR_INVALID = 1

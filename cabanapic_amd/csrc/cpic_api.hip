@@ -52,7 +52,7 @@ struct cpic_context {
   long nx = 0, ny = 0, nz = 0;
   long ncells = 0;  // Ng
   long nc = 0;      // interior cells
-  long nc_pad = 0;  // interior cells rounded up to whole groups of 64
+  long nc_pad = 0;  // interior cells rounded up to whole blocks (kBlock lanes)
   long cap = 0;     // particle capacity declared at creation
   long np = 0;      // particles resident (conserved by the push)
   Grid grid{};
@@ -76,6 +76,7 @@ struct cpic_context {
   unsigned int* ovf_count = nullptr;
   int ovf_cap = 0;
   int* max_count = nullptr;  // largest column occupancy seen by the last push
+  unsigned long long* dep = nullptr;  // departure masks of pass 1, [6][MW][nc_pad]
 
   void* fields = nullptr;
   void* interp = nullptr;
@@ -229,14 +230,13 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   if ((s = fill_zero<int>(c, c->max_count, 1))) return s;
   const int blocks = cdiv(c->nc, kBlock);
   Overflow<R> ovf{(R*)c->ovf_rec, c->ovf_cell, c->ovf_count, c->ovf_cap};
-  RecordList<R> L1{(R*)c->rec1, c->rc1, c->M1};
-  RecordList<R> L2{(R*)c->rec2, c->rc2, c->M2};
+  RecordList<R> L1{(R*)c->rec1, c->rc1, c->M1, (int)c->nc};
+  RecordList<R> L2{(R*)c->rec2, c->rc2, c->M2, (int)c->nc};
   const ColState<R> cs = col_state<R>(c, c->cur);
 
   PushPass<R> p;
   p.cs = cs;
-  p.out = L1;
-  p.ovf = ovf;
+  p.dep = c->dep;
   p.interp = (const R*)c->interp;
   p.acc = (R*)c->acc;
   p.g = c->cg;
@@ -254,10 +254,27 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   k_push_pass<R><<<blocks, kBlock, 0, c->stream>>>(p);
   CHECK_LAUNCH();
 
-  // the streak after the 1st, 2nd and 3rd crossings
-  const RecordList<R> ins[3] = {L1, L2, L1};
-  const RecordList<R> outs[3] = {L2, L1, L2};
-  for (int pass = 0; pass < 3; ++pass) {
+  // the streak after the 1st crossing, found through the departure masks
+  ArrivePass<R> ar;
+  ar.cs = cs;
+  ar.dep = c->dep;
+  ar.out = L2;
+  ar.ovf = ovf;
+  ar.acc = (R*)c->acc;
+  ar.g = c->cg;
+  ar.ncells = (int)c->ncells;
+  ar.cdt_dx = cdt_dx;
+  ar.cdt_dy = cdt_dy;
+  ar.cdt_dz = cdt_dz;
+  ar.qsp = qsp;
+  ar.err = c->err;
+  k_arrive_pass<R><<<blocks, kBlock, 0, c->stream>>>(ar);
+  CHECK_LAUNCH();
+
+  // the streaks after the 2nd and 3rd crossings, from the record lists
+  const RecordList<R> ins[2] = {L2, L1};
+  const RecordList<R> outs[2] = {L1, L2};
+  for (int pass = 0; pass < 2; ++pass) {
     FinishPass<R> f;
     f.cs = cs;
     f.in = ins[pass];
@@ -267,7 +284,7 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
     f.g = c->cg;
     f.ncells = (int)c->ncells;
     f.qsp = qsp;
-    f.last = (pass == 2) ? 1 : 0;
+    f.last = (pass == 1) ? 1 : 0;
     f.err = c->err;
     k_finish_pass<R><<<blocks, kBlock, 0, c->stream>>>(f);
     CHECK_LAUNCH();
@@ -388,7 +405,7 @@ int ensure_columns(cpic_context* c, int D, int M1, int M2) {
     c->mask[b] = nullptr;
   }
   void** recs[] = {&c->rec1, &c->rec2, (void**)&c->rc1, (void**)&c->rc2, &c->ovf_rec,
-                   (void**)&c->ovf_cell};
+                   (void**)&c->ovf_cell, (void**)&c->dep};
   for (void** p : recs) {
     if (*p) HIP_TRY(hipFree(*p));
     *p = nullptr;
@@ -402,17 +419,20 @@ int ensure_columns(cpic_context* c, int D, int M1, int M2) {
     if ((s = alloc((void**)&c->mask[b], sizeof(unsigned long long) * c->nc_pad * ((D + 63) / 64))))
       return s;
   }
-  const size_t per = (size_t)kRecordLen * c->rsz() * 6 * (size_t)c->nc;
+  const size_t per = (size_t)record_bytes(c->real_size) * 6 * (size_t)c->nc;
   if ((s = alloc(&c->rec1, per * M1))) return s;
   if ((s = alloc(&c->rec2, per * M2))) return s;
   if ((s = alloc((void**)&c->rc1, sizeof(int) * 6 * c->nc))) return s;
   if ((s = alloc((void**)&c->rc2, sizeof(int) * 6 * c->nc))) return s;
   c->ovf_cap = (int)std::max(65536L, c->nc / 4);
-  if ((s = alloc(&c->ovf_rec, (size_t)kRecordLen * c->rsz() * c->ovf_cap))) return s;
+  if ((s = alloc(&c->ovf_rec, (size_t)record_bytes(c->real_size) * c->ovf_cap))) return s;
   if ((s = alloc((void**)&c->ovf_cell, sizeof(int) * c->ovf_cap))) return s;
   if ((s = fill_zero<int>(c, c->cnt[0], c->nc_pad))) return s;
   if ((s = fill_zero<int>(c, c->cnt[1], c->nc_pad))) return s;
   const int MW = (D + 63) / 64;
+  if ((s = alloc((void**)&c->dep, sizeof(unsigned long long) * 6 * (size_t)MW * c->nc_pad)))
+    return s;
+  if ((s = fill_zero<unsigned long long>(c, c->dep, 6 * (size_t)MW * c->nc_pad))) return s;
   if ((s = fill_zero<unsigned long long>(c, c->mask[0], (size_t)MW * c->nc_pad))) return s;
   if ((s = fill_zero<unsigned long long>(c, c->mask[1], (size_t)MW * c->nc_pad))) return s;
   c->D = D;
@@ -476,7 +496,7 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
   c->grid.sz = (int)(X * Y);
   c->grid.ncells = (int)c->ncells;
   c->nc = cfg->nx * cfg->ny * cfg->nz;
-  c->nc_pad = (c->nc + 63) / 64 * 64;
+  c->nc_pad = (c->nc + kBlock - 1) / kBlock * kBlock;
   c->cg.nx = (int)c->nx;
   c->cg.ny = (int)c->ny;
   c->cg.nz = (int)c->nz;
@@ -536,7 +556,7 @@ int cpic_destroy(cpic_context* c) {
   void* ptrs[] = {c->rec1,   c->rec2,     c->rc1,      c->rc2,     c->ovf_rec,
                   c->ovf_cell, c->ovf_count, c->max_count, c->fields, c->interp,
                   c->acc,    c->partials, c->red_out,  c->err,     c->movers,
-                  c->hist_dev};
+                  c->hist_dev, c->dep};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

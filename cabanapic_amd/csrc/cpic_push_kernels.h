@@ -16,37 +16,47 @@
 //
 // Occupancy.  A column uses rows [0, len) and a bit mask per cell marks the
 // occupied rows (words mask[w * Nc_pad + ci], bit r of word w = row 64w + r).
-// The push normally keeps a particle that stays in its cell in its row, so
-// the stores of a row stay one contiguous access; the rows of particles that
-// left become holes, which the particles arriving from the neighbours fill
-// before the column grows.  Every few steps a compacting push packs each
-// column again (stayers to rows 0, 1, 2, ...).
+// The push normally keeps every particle in its row, so the stores of a row
+// stay one contiguous access; the rows of particles that left become holes,
+// which the particles arriving from the neighbours fill before the column
+// grows.  Every few steps a compacting push packs each column again (rows
+// 0, 1, 2, ...).
 //
 // Cell crossings.  The reference's move_p loop (src/move_p.h:93-371) walks a
 // particle through up to four streaks, each in one cell.  Here every streak
 // is executed by the lane that owns the streak's cell:
 //   pass 1   pushes every particle of the cell (src/push.h:65-295) and runs
-//            its first streak.  A particle that ends in the cell is written
-//            to the output column ("stayer").  A particle that crosses face
-//            d is written as a record (position on the entry face of the
-//            neighbour, remaining displacement, momentum, weight) into slot
-//            k of the (cell, d) record list.  The list of one (cell, d) is
-//            contiguous, so each lane writes and later reads its records
-//            sequentially.
-//   pass 2-4 the owner of the neighbour cell reads the records addressed to
-//            it from its six neighbours, runs the next streak, and either
-//            appends the particle to its own column or forwards a new record.
-// The arithmetic of every streak is the reference's, so positions, momenta
-// and cells are bit-identical to one uninterrupted move_p.  Record lists
-// that fill up spill into an overflow list that a final fallback kernel
-// finishes with atomic deposits (rare by construction of the capacities).
+//            its first streak.  Every particle is stored in its own row (a
+//            contiguous 64-wide access): one that stays with its new
+//            position, one that leaves through face d with its position
+//            before the push and its new momentum.  A departure keeps its
+//            row reserved in the mask and sets a bit in the departure mask of
+//            face d.
+//   pass 2   the owner of the neighbour cell finds the departures through
+//            its neighbours' departure masks, reads them from their rows,
+//            recomputes displacement and first streak with the same
+//            operations (bit-identical), and runs the next streak; the
+//            particle is appended to its column or written as a record.
+//            Each cell then releases the rows of its own departures.
+//   pass 3-4 the streaks after the 2nd and 3rd crossings, from record lists
+//            read by the owner of the cell each record entered.
+// Writing departures in place instead of as scattered records keeps every
+// write of pass 1 a full-row access.  The arithmetic of every streak is the
+// reference's, so positions, momenta and cells are bit-identical to one
+// uninterrupted move_p.  Record lists that fill up spill into an overflow
+// list that a final fallback kernel finishes with atomic deposits (rare by
+// construction of the capacities).
 #pragma once
 
 #include "cpic_kernels.h"
 
 namespace cpic {
 
-constexpr int kRecordLen = 12;  // pos[3], rem[3], u[3], w, 2 pad
+// A crossing record: pos[3], rem[3], u[3], w, then padding to whole 32-byte
+// sectors (64 bytes in float, 96 in double), so that a record write never
+// leaves a partially written sector, which memory would have to read back.
+template <class R> struct RecordLen { static constexpr int value = sizeof(R) == 4 ? 16 : 12; };
+constexpr int record_bytes(int real_size) { return real_size == 4 ? 64 : 96; }
 constexpr int kSlotLen = 8;     // x, y, z, ux, uy, uz, w, pad
 
 __device__ __forceinline__ size_t col_slot(int ci, int row, int D) {
@@ -94,6 +104,11 @@ __device__ __forceinline__ int wave_sum_int(int v) {
   return v;
 }
 
+// Marks a loaded vector as consumed at this point (an empty asm that reads
+// and writes its registers): the wait for the load is placed here, and the
+// load cannot be moved past it.
+template <class V> __device__ __forceinline__ void settle(V& v) { asm volatile("" : "+v"(v)); }
+
 // correction term of a streak (move_p.h:154): real_t product scaled by the
 // double 1/3, rounded back to real_t
 __device__ __forceinline__ float streak_v5(float q, float a, float b, float c) {
@@ -106,6 +121,57 @@ __device__ __forceinline__ double streak_v5(double q, double a, double b, double
 // sqrtf of push.h:148,169 narrows a double argument to float
 __device__ __forceinline__ float sqrtf_narrow(float v) { return sqrtf(v); }
 __device__ __forceinline__ double sqrtf_narrow(double v) { return (double)sqrtf((float)v); }
+
+// Field gather and relativistic Boris push with the tan-series rotation,
+// push.h:124-163, for a particle at (dx, dy, dz) of a cell whose
+// interpolator is (e0, e1, e2, b0, b1); updates (ux, uy, uz).
+template <class V4, class R>
+__device__ __forceinline__ void boris(const V4& e0, const V4& e1, const V4& e2, const V4& b0,
+                                      const V4& b1, R qdt_2mc, R dx, R dy, R dz, R& ux, R& uy,
+                                      R& uz) {
+  const R one = 1.;
+  const R one_third = 1. / 3.;
+  const R two_fifteenths = 2. / 15.;
+  const R hax = qdt_2mc * ((e0.x + dy * e0.y) + dz * (e0.z + dy * e0.w));
+  const R hay = qdt_2mc * ((e1.x + dz * e1.y) + dx * (e1.z + dz * e1.w));
+  const R haz = qdt_2mc * ((e2.x + dx * e2.y) + dy * (e2.z + dx * e2.w));
+  const R cbx = b0.x + dx * b0.y;
+  const R cby = b0.z + dy * b0.w;
+  const R cbz = b1.x + dz * b1.y;
+  ux += hax;
+  uy += hay;
+  uz += haz;
+  R v0 = qdt_2mc / sqrtf_narrow(one + (ux * ux + (uy * uy + uz * uz)));
+  R v1 = cbx * cbx + (cby * cby + cbz * cbz);
+  R v2 = (v0 * v0) * v1;
+  R v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
+  R v4 = v3 / (one + v1 * (v3 * v3));
+  v4 += v4;
+  v0 = ux + v3 * (uy * cbz - uz * cby);
+  v1 = uy + v3 * (uz * cbx - ux * cbz);
+  v2 = uz + v3 * (ux * cby - uy * cbx);
+  ux += v4 * (v1 * cbz - v2 * cby);
+  uy += v4 * (v2 * cbx - v0 * cbz);
+  uz += v4 * (v0 * cby - v1 * cbx);
+  ux += hax;
+  uy += hay;
+  uz += haz;
+}
+
+// Displacement in cell units of the pushed momentum, push.h:169-182; pass 2
+// recomputes it from the stored momentum with the same operations.
+template <class R>
+__device__ __forceinline__ void displacement(R ux, R uy, R uz, R cdt_dx, R cdt_dy, R cdt_dz,
+                                             R& ddx, R& ddy, R& ddz) {
+  const R one = 1.;
+  const R v0 = one / sqrtf_narrow(one + (ux * ux + (uy * uy + uz * uz)));
+  ddx = ux * cdt_dx;
+  ddy = uy * cdt_dy;
+  ddz = uz * cdt_dz;
+  ddx *= v0;
+  ddy *= v0;
+  ddz *= v0;
+}
 
 // The four quadrant contributions of one current component for a streak
 // with displacement dA along the component axis and midpoint (mB, mC) on
@@ -205,11 +271,21 @@ template <class R> struct RowCursor {
   int len, hw;
   unsigned long long word, freeb;
   bool have, append;
-  __device__ void start(int l) {
+  // l: rows in use; w0: mask word 0 of the column (read by the caller
+  // together with its other loads)
+  __device__ void start(int l, unsigned long long w0) {
     len = l;
-    hw = -1;
-    word = freeb = 0;
-    have = append = false;
+    append = false;
+    if (l > 0) {
+      hw = 0;
+      word = w0;
+      freeb = ~w0 & low_bits(l);
+      have = true;
+    } else {
+      hw = -1;
+      word = freeb = 0;
+      have = false;
+    }
   }
   __device__ int next(const ColState<R>& cs, int ci) {
     for (int guard = 0; guard < 64; ++guard) {
@@ -252,15 +328,18 @@ template <class R> struct RowCursor {
   }
 };
 
+// Record list k-th slot of (cell, d): [d][k][cell], so the records that
+// neighbouring lanes write or read for the same (d, k) are adjacent.
 template <class R> struct RecordList {
-  R* rec;    // [Nc][6][M] records of kRecordLen
+  R* rec;    // [6][M][Nc] records of RecordLen<R> reals
   int* cnt;  // [6][Nc]
   int M;
+  int nc;
 };
 
 template <class R>
 __device__ __forceinline__ R* record_at(const RecordList<R>& L, int ci, int d, int k) {
-  return L.rec + ((((size_t)ci * 6 + d) * L.M) + k) * kRecordLen;
+  return L.rec + ((((size_t)d * L.M) + k) * (size_t)L.nc + ci) * RecordLen<R>::value;
 }
 
 template <class R> struct Overflow {
@@ -290,11 +369,12 @@ __device__ __forceinline__ void put_record(const RecordList<R>& L, const Overflo
       return;
     }
     ovf.cell[slot] = face_neighbor(g, ci, d);
-    dst = reinterpret_cast<V4*>(ovf.rec + (size_t)slot * kRecordLen);
+    dst = reinterpret_cast<V4*>(ovf.rec + (size_t)slot * RecordLen<R>::value);
   }
   dst[0] = a;
   dst[1] = b;
   dst[2] = c;
+  if (RecordLen<R>::value == 16) dst[3] = V4{};
   ++k;
 }
 
@@ -310,26 +390,41 @@ __device__ __forceinline__ void store_particle(const ColState<R>& cs, int ci, in
   dst[1] = b;
 }
 
+constexpr int kRowBatch = 2;  // rows loaded together by pass 1 (divides 64)
+
+// Departure masks: bit r of word w of (face d, cell c) is set when the
+// particle stored in row 64w + r of cell c's output column left the cell
+// through face d in pass 1.  Layout [d][w][nc_pad].
+__device__ __forceinline__ size_t dep_index(int d, int w, int MW, size_t MP, int ci) {
+  return ((size_t)d * (size_t)MW + (size_t)w) * MP + (size_t)ci;
+}
+
 template <class R> struct PushPass {
   ColState<R> cs;
-  RecordList<R> out;  // records of first crossings
-  Overflow<R> ovf;
+  unsigned long long* dep;  // departure masks written by this pass
   const R* interp;
   R* acc;  // slot-major [12][Ng]
   CellGrid g;
   int ncells;  // Ng
   R qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
   int accumulate;  // 1: acc += (push semantics), 0: acc = (after a clear)
-  int compact;     // 1: pack stayers to rows 0,1,..; 0: keep stayers in their row
+  int compact;     // 1: pack the rows to 0,1,..; 0: keep every particle in its row
   int* err;
   unsigned* movers;
   int* max_count;
 };
 
 // Pass 1: push + first streak, one lane per cell (src/push.h:65-295).
+// A particle that stays is stored with its new position.  A particle that
+// leaves through face d (after depositing its first streak here) is stored
+// in the same way but with its position before the push, its new momentum
+// and its weight: the owner of the neighbour cell recomputes the
+// displacement and the first streak from these in pass 2.  Its row is
+// marked in the occupancy mask (reserved until pass 2 has read it) and in
+// the departure mask of face d.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
-  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
   const bool valid = ci < a.g.nc;
   const int len = valid ? a.cs.cnt_in[ci] : 0;
   const int lmax = wave_max_int(len);
@@ -346,64 +441,42 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   }
   const R one = 1.;
   const R one_third = 1. / 3.;
-  const R two_fifteenths = 2. / 15.;
   const R qdt_2mc = a.qdt_2mc;
   R acc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0;
   int ns = 0, hi = 0, nmov = 0;
-  int kd[6] = {0, 0, 0, 0, 0, 0};
   unsigned long long m_in = 0, m_out = 0;
   const size_t MP = a.cs.nc_pad;
+  const int MW = a.cs.MW;
+  // departure bits of output word cw, one word per face
+  int cw = 0;
+  unsigned long long dw0 = 0, dw1 = 0, dw2 = 0, dw3 = 0, dw4 = 0, dw5 = 0;
+  auto flush_dep = [&]() {
+    a.dep[dep_index(0, cw, MW, MP, ci)] = dw0;
+    a.dep[dep_index(1, cw, MW, MP, ci)] = dw1;
+    a.dep[dep_index(2, cw, MW, MP, ci)] = dw2;
+    a.dep[dep_index(3, cw, MW, MP, ci)] = dw3;
+    a.dep[dep_index(4, cw, MW, MP, ci)] = dw4;
+    a.dep[dep_index(5, cw, MW, MP, ci)] = dw5;
+    dw0 = dw1 = dw2 = dw3 = dw4 = dw5 = 0;
+  };
 
-  for (int j = 0; j < lmax; ++j) {
+  auto process = [&](int j, bool have, const V4& pa, const V4& pb) {
     const int b = j & 63;
-    if (b == 0) m_in = valid ? a.cs.mask_in[(size_t)(j >> 6) * MP + ci] : 0ull;
-    if (j < len && ((m_in >> b) & 1ull)) {
-      const V4* src = reinterpret_cast<const V4*>(a.cs.in + col_slot(ci, j, a.cs.D) * kSlotLen);
-      const V4 pa = src[0], pb = src[1];
+    if (have) {
       const R dx = pa.x, dy = pa.y, dz = pa.z;
       R ux = pa.w, uy = pb.x, uz = pb.y;
       const R w = pb.z;
-      // field gather, push.h:124-138
-      const R hax = qdt_2mc * ((e0.x + dy * e0.y) + dz * (e0.z + dy * e0.w));
-      const R hay = qdt_2mc * ((e1.x + dz * e1.y) + dx * (e1.z + dz * e1.w));
-      const R haz = qdt_2mc * ((e2.x + dx * e2.y) + dy * (e2.z + dx * e2.w));
-      const R cbx = b0.x + dx * b0.y;
-      const R cby = b0.z + dy * b0.w;
-      const R cbz = b1.x + dz * b1.y;
-      ux += hax;
-      uy += hay;
-      uz += haz;
-      // Boris rotation with the tan series, push.h:148-163
-      R v0 = qdt_2mc / sqrtf_narrow(one + (ux * ux + (uy * uy + uz * uz)));
-      R v1 = cbx * cbx + (cby * cby + cbz * cbz);
-      R v2 = (v0 * v0) * v1;
-      R v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
-      R v4 = v3 / (one + v1 * (v3 * v3));
-      v4 += v4;
-      v0 = ux + v3 * (uy * cbz - uz * cby);
-      v1 = uy + v3 * (uz * cbx - ux * cbz);
-      v2 = uz + v3 * (ux * cby - uy * cbx);
-      ux += v4 * (v1 * cbz - v2 * cby);
-      uy += v4 * (v2 * cbx - v0 * cbz);
-      uz += v4 * (v0 * cby - v1 * cbx);
-      ux += hax;
-      uy += hay;
-      uz += haz;
+      boris(e0, e1, e2, b0, b1, qdt_2mc, dx, dy, dz, ux, uy, uz);
       // displacement in cell units, push.h:169-182
-      v0 = one / sqrtf_narrow(one + (ux * ux + (uy * uy + uz * uz)));
-      R ddx = ux * a.cdt_dx;
-      R ddy = uy * a.cdt_dy;
-      R ddz = uz * a.cdt_dz;
-      ddx *= v0;
-      ddy *= v0;
-      ddz *= v0;
+      R ddx, ddy, ddz;
+      displacement(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, ddx, ddy, ddz);
       const R mx = dx + ddx, my = dy + ddy, mz = dz + ddz;
       const R nx = mx + ddx, ny = my + ddy, nz = mz + ddz;
       const R q = w * a.qsp;
       R px, py, pz;
-      bool stays;
+      int d = -1;
       if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
         const R v5 = q * ddx * ddy * ddz * one_third;
         R o[12];
@@ -413,41 +486,84 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
         px = nx;
         py = ny;
         pz = nz;
-        stays = true;
       } else {
         ++nmov;
         R pos[3] = {dx, dy, dz};
         R rem[3] = {ddx, ddy, ddz};
-        const int d = streak(pos, rem, q, acc);
-        px = pos[0];
-        py = pos[1];
-        pz = pos[2];
-        stays = d < 0;
-        if (!stays) {
-          int k = kd[d];
-          put_record(a.out, a.ovf, a.g, ci, d, k, pos, rem, ux, uy, uz, w, a.err);
-          kd[d] = k;
-        }
+        d = streak(pos, rem, q, acc);
+        // a particle that leaves is stored with its position before the push
+        px = d < 0 ? pos[0] : dx;
+        py = d < 0 ? pos[1] : dy;
+        pz = d < 0 ? pos[2] : dz;
       }
-      if (stays) {
-        int row;
-        if (a.compact) {
-          row = ns++;
-        } else {
-          row = j;
-          m_out |= 1ull << b;
-          hi = j + 1;
+      int row;
+      if (a.compact) {
+        row = ns++;
+        if ((row >> 6) != cw) {
+          flush_dep();
+          cw = row >> 6;
         }
-        store_particle(a.cs, ci, row, px, py, pz, ux, uy, uz, w);
+      } else {
+        row = j;
+        m_out |= 1ull << b;
+        hi = j + 1;
       }
+      if (d >= 0) {
+        const unsigned long long bit = 1ull << (row & 63);
+        dw0 |= d == 0 ? bit : 0ull;
+        dw1 |= d == 1 ? bit : 0ull;
+        dw2 |= d == 2 ? bit : 0ull;
+        dw3 |= d == 3 ? bit : 0ull;
+        dw4 |= d == 4 ? bit : 0ull;
+        dw5 |= d == 5 ? bit : 0ull;
+      }
+      store_particle(a.cs, ci, row, px, py, pz, ux, uy, uz, w);
     }
     if (!a.compact && (b == 63 || j == lmax - 1)) {
-      if (valid) a.cs.mask_out[(size_t)(j >> 6) * MP + ci] = m_out;
+      a.cs.mask_out[(size_t)(j >> 6) * MP + ci] = m_out;
       m_out = 0;
+      flush_dep();
+      cw = (j >> 6) + 1;
     }
+  };
+
+  // Rows are read kRowBatch at a time, all loads of a batch issued before
+  // any store of it, so one wait for memory covers the whole batch (vector
+  // loads and stores retire in order, so a wait for a load also waits for
+  // the stores issued before it).  Every lane issues every load: lanes
+  // without a particle in the row read one shared slot instead.
+  const R* dummy = a.cs.in;
+  for (int j0 = 0; j0 < lmax; j0 += kRowBatch) {
+    if ((j0 & 63) == 0) {
+      m_in = (valid && j0 < len) ? a.cs.mask_in[(size_t)(j0 >> 6) * MP + ci] : 0ull;
+      settle(m_in);  // consume the word here, once per 64 rows
+    }
+    V4 pa[kRowBatch], pb[kRowBatch];
+    bool have[kRowBatch];
+#pragma unroll
+    for (int u = 0; u < kRowBatch; ++u) {
+      const int j = j0 + u;
+      have[u] = valid && j < len && ((m_in >> (j & 63)) & 1ull);
+      const V4* src = reinterpret_cast<const V4*>(
+          have[u] ? a.cs.in + col_slot(ci, j, a.cs.D) * kSlotLen : dummy);
+      pa[u] = src[0];
+      pb[u] = src[1];
+    }
+    // one wait for the whole batch, here: the values are marked as used, so
+    // the compiler neither moves the loads into the per-row branches nor
+    // waits again inside them
+#pragma unroll
+    for (int u = 0; u < kRowBatch; ++u) {
+      settle(pa[u]);
+      settle(pb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kRowBatch; ++u)
+      if (j0 + u < lmax) process(j0 + u, have[u], pa[u], pb[u]);
   }
+
+  const size_t N = a.ncells;
   if (valid) {
-    const size_t N = a.ncells;
     if (a.accumulate) {
 #pragma unroll
       for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc[k];
@@ -455,15 +571,20 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
 #pragma unroll
       for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc[k];
     }
-    if (a.compact) {
-      for (int w = 0; w < a.cs.MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = low_bits(ns - 64 * w);
-      a.cs.cnt_out[ci] = ns;
-    } else {
-      for (int w = (lmax + 63) >> 6; w < a.cs.MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
-      a.cs.cnt_out[ci] = hi;
+  }
+  // the remaining mask and departure words of the column
+  if (a.compact) {
+    flush_dep();
+    for (int w = 0; w < MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = low_bits(ns - 64 * w);
+    for (int w = cw + 1; w < MW; ++w)
+      for (int d = 0; d < 6; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
+    a.cs.cnt_out[ci] = ns;
+  } else {
+    for (int w = (lmax + 63) >> 6; w < MW; ++w) {
+      a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
+      for (int d = 0; d < 6; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
     }
-#pragma unroll
-    for (int d = 0; d < 6; ++d) a.out.cnt[(size_t)d * a.g.nc + ci] = min(kd[d], a.out.M);
+    a.cs.cnt_out[ci] = hi;
   }
   const int wm = wave_sum_int(nmov);
   if ((threadIdx.x & 63) == 0) {
@@ -471,6 +592,8 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
     atomicMax(a.max_count, lmax);
   }
 }
+
+constexpr int kRecBatch = 2;  // records loaded together by the finish passes
 
 template <class R> struct FinishPass {
   ColState<R> cs;  // cs.out / cs.cnt_out: the columns being filled
@@ -520,40 +643,67 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   if (tmax > 0) {
     const int v = valid ? interior_to_voxel(a.g, ci) : 0;
     RowCursor<R> rows;
-    rows.start(valid ? a.cs.cnt_out[ci] : 0);
+    {
+      const int l = valid ? a.cs.cnt_out[ci] : 0;
+      unsigned long long w0 = valid ? a.cs.mask_out[ci] : 0ull;
+      asm volatile("" : "+v"(w0));  // wait for the word here, before the loop
+      rows.start(l, w0);
+    }
     R acc[12];
 #pragma unroll
     for (int k = 0; k < 12; ++k) acc[k] = 0;
     typedef typename Vec4T<R>::type V4;
+    // (d, k): the next record, walked in direction then record order.  Every
+    // lane issues every load of a batch (lanes past their last record read
+    // one shared record), so one wait covers the batch.
     int d = 0, k = 0;
-    for (int i = 0; i < tmax; ++i) {
+    const R* rdummy = a.in.rec;
+    auto request = [&](int i, V4& r0, V4& r1, V4& r2) {
+      const R* p = rdummy;
       if (i < total) {
-        while (d < 6 && k >= sel6(d, c0, c1, c2, c3, c4, c5)) {
+        while (d < 5 && k >= sel6(d, c0, c1, c2, c3, c4, c5)) {
           k = 0;
           ++d;
         }
-        if (d >= 6) break;  // unreachable while the counts are consistent
-        const int si = sel6(d, s0, s1, s2, s3, s4, s5);
-        const V4* src = reinterpret_cast<const V4*>(record_at(a.in, si, d, k));
+        p = record_at(a.in, sel6(d, s0, s1, s2, s3, s4, s5), d, k);
         ++k;
-        const V4 r0 = src[0], r1 = src[1], r2 = src[2];
-        R pos[3] = {r0.x, r0.y, r0.z};
-        R rem[3] = {r0.w, r1.x, r1.y};
-        const R ux = r1.z, uy = r1.w, uz = r2.x, w = r2.y;
-        const R q = w * a.qsp;
-        const int d2 = streak(pos, rem, q, acc);
-        if (d2 < 0) {
-          const int row = rows.next(a.cs, ci);
-          if (row >= 0)
-            store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w);
-          else
-            atomicOr(a.err, kErrColumnFull);
-        } else {
-          int kk = a.last ? a.out.M : kd[d2];  // last pass: straight to overflow
-          put_record(a.out, a.ovf, a.g, ci, d2, kk, pos, rem, ux, uy, uz, w, a.err);
-          if (!a.last) kd[d2] = kk;
-        }
       }
+      const V4* src = reinterpret_cast<const V4*>(p);
+      r0 = src[0];
+      r1 = src[1];
+      r2 = src[2];
+    };
+    auto process = [&](const V4& r0, const V4& r1, const V4& r2) {
+      R pos[3] = {r0.x, r0.y, r0.z};
+      R rem[3] = {r0.w, r1.x, r1.y};
+      const R ux = r1.z, uy = r1.w, uz = r2.x, w = r2.y;
+      const R q = w * a.qsp;
+      const int d2 = streak(pos, rem, q, acc);
+      if (d2 < 0) {
+        const int row = rows.next(a.cs, ci);
+        if (row >= 0)
+          store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w);
+        else
+          atomicOr(a.err, kErrColumnFull);
+      } else {
+        int kk = a.last ? a.out.M : kd[d2];  // last pass: straight to overflow
+        put_record(a.out, a.ovf, a.g, ci, d2, kk, pos, rem, ux, uy, uz, w, a.err);
+        if (!a.last) kd[d2] = kk;
+      }
+    };
+    for (int i0 = 0; i0 < tmax; i0 += kRecBatch) {
+      V4 r0[kRecBatch], r1[kRecBatch], r2[kRecBatch];
+#pragma unroll
+      for (int u = 0; u < kRecBatch; ++u) request(i0 + u, r0[u], r1[u], r2[u]);
+#pragma unroll
+      for (int u = 0; u < kRecBatch; ++u) {
+        settle(r0[u]);
+        settle(r1[u]);
+        settle(r2[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kRecBatch; ++u)
+        if (i0 + u < total) process(r0[u], r1[u], r2[u]);
     }
     if (valid) {
       rows.finish(a.cs, ci);
@@ -566,6 +716,158 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
     }
   }
   if (valid && !a.last) {
+#pragma unroll
+    for (int dd = 0; dd < 6; ++dd) a.out.cnt[(size_t)dd * NC + ci] = min(kd[dd], a.out.M);
+  }
+}
+
+template <class R> struct ArrivePass {
+  ColState<R> cs;                        // cs.out: the columns of pass 1
+  const unsigned long long* dep;         // departure masks of pass 1
+  RecordList<R> out;                     // records of the next crossings
+  Overflow<R> ovf;
+  R* acc;
+  CellGrid g;
+  int ncells;
+  R cdt_dx, cdt_dy, cdt_dz, qsp;
+  int* err;
+};
+
+// Pass 2: every particle that left a cell in pass 1 continues in the cell
+// across the face it crossed.  The owner of that cell finds it through the
+// departure masks of its six neighbours, reads it from the neighbour's
+// column, recomputes its displacement and first streak (the same operations
+// as pass 1, without the deposit, which pass 1 made), and runs the next
+// streak.  Afterwards each cell releases the rows of its own departures.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
+  const bool valid = ci < a.g.nc;
+  const size_t MP = a.cs.nc_pad;
+  const int MW = a.cs.MW;
+  const size_t NC = a.g.nc;
+  typedef typename Vec4T<R>::type V4;
+  // a particle that crossed face d came from the neighbour across the
+  // opposite face
+  int s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+  if (valid) {
+    s0 = face_neighbor(a.g, ci, 3);
+    s1 = face_neighbor(a.g, ci, 4);
+    s2 = face_neighbor(a.g, ci, 5);
+    s3 = face_neighbor(a.g, ci, 0);
+    s4 = face_neighbor(a.g, ci, 1);
+    s5 = face_neighbor(a.g, ci, 2);
+  }
+  const int v = valid ? interior_to_voxel(a.g, ci) : 0;
+  RowCursor<R> rows;
+  {
+    const int l = valid ? a.cs.cnt_out[ci] : 0;
+    unsigned long long w0 = valid ? a.cs.mask_out[ci] : 0ull;
+    settle(w0);
+    rows.start(l, w0);
+  }
+  R acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0;
+  int kd[6] = {0, 0, 0, 0, 0, 0};
+  int total_all = 0;
+  const R* dummy = a.cs.out;
+
+  for (int w = 0; w < MW; ++w) {
+    unsigned long long d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0;
+    if (valid) {
+      d0 = a.dep[dep_index(0, w, MW, MP, s0)];
+      d1 = a.dep[dep_index(1, w, MW, MP, s1)];
+      d2 = a.dep[dep_index(2, w, MW, MP, s2)];
+      d3 = a.dep[dep_index(3, w, MW, MP, s3)];
+      d4 = a.dep[dep_index(4, w, MW, MP, s4)];
+      d5 = a.dep[dep_index(5, w, MW, MP, s5)];
+    }
+    const int total = __popcll(d0) + __popcll(d1) + __popcll(d2) + __popcll(d3) +
+                      __popcll(d4) + __popcll(d5);
+    total_all += total;
+    const int tmax = wave_max_int(total);
+    if (tmax == 0) continue;
+    // the next departure to read: face d, bits left in the current word
+    int d = 0;
+    unsigned long long bits = d0;
+    auto request = [&](int i, V4& x, V4& y) {
+      const R* src = dummy;
+      if (i < total) {
+        while (bits == 0 && d < 5) {
+          ++d;
+          bits = d == 1 ? d1 : d == 2 ? d2 : d == 3 ? d3 : d == 4 ? d4 : d5;
+        }
+        if (bits != 0) {  // always, while the counts are consistent
+          const int r = 64 * w + __ffsll((long long)bits) - 1;
+          bits &= bits - 1;
+          const int sc = sel6(d, s0, s1, s2, s3, s4, s5);
+          src = a.cs.out + col_slot(sc, r, a.cs.D) * kSlotLen;
+        } else {
+          atomicOr(a.err, kErrBadCell);
+        }
+      }
+      const V4* p = reinterpret_cast<const V4*>(src);
+      x = p[0];
+      y = p[1];
+    };
+    auto process = [&](const V4& pa, const V4& pb) {
+      const R ux = pa.w, uy = pb.x, uz = pb.y, w8 = pb.z;
+      R pos[3] = {pa.x, pa.y, pa.z};
+      R rem[3];
+      displacement(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, rem[0], rem[1], rem[2]);
+      const R q = w8 * a.qsp;
+      R scratch[12];
+#pragma unroll
+      for (int k = 0; k < 12; ++k) scratch[k] = 0;
+      // the first streak, deposited by the neighbour in pass 1
+      if (streak(pos, rem, q, scratch) < 0) atomicOr(a.err, kErrBadCell);
+      const int d2n = streak(pos, rem, q, acc);
+      if (d2n < 0) {
+        const int row = rows.next(a.cs, ci);
+        if (row >= 0)
+          store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w8);
+        else
+          atomicOr(a.err, kErrColumnFull);
+      } else {
+        int kk = kd[d2n];
+        put_record(a.out, a.ovf, a.g, ci, d2n, kk, pos, rem, ux, uy, uz, w8, a.err);
+        kd[d2n] = kk;
+      }
+    };
+    for (int i0 = 0; i0 < tmax; i0 += kRecBatch) {
+      V4 x[kRecBatch], y[kRecBatch];
+#pragma unroll
+      for (int u = 0; u < kRecBatch; ++u) request(i0 + u, x[u], y[u]);
+#pragma unroll
+      for (int u = 0; u < kRecBatch; ++u) {
+        settle(x[u]);
+        settle(y[u]);
+      }
+#pragma unroll
+      for (int u = 0; u < kRecBatch; ++u)
+        if (i0 + u < total) process(x[u], y[u]);
+    }
+  }
+
+  if (valid) {
+    rows.finish(a.cs, ci);
+    a.cs.cnt_out[ci] = rows.len;
+    if (total_all > 0) {
+      const size_t N = a.ncells;
+#pragma unroll
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc[k];
+    }
+    // release the rows of this cell's own departures: only their mask bits
+    // are cleared here.  Other lanes may still be reading those rows in this
+    // pass, but no lane writes them before the next pass, which is the first
+    // that can hand them out again.
+    for (int w = 0; w < MW; ++w) {
+      unsigned long long own = 0;
+#pragma unroll
+      for (int dd = 0; dd < 6; ++dd) own |= a.dep[dep_index(dd, w, MW, MP, ci)];
+      if (own) a.cs.mask_out[(size_t)w * MP + ci] &= ~own;
+    }
 #pragma unroll
     for (int dd = 0; dd < 6; ++dd) a.out.cnt[(size_t)dd * NC + ci] = min(kd[dd], a.out.M);
   }
@@ -589,7 +891,7 @@ __global__ void __launch_bounds__(kBlock) k_spill_pass(SpillPass<R> a) {
   const unsigned total = min(*a.ovf.count, (unsigned)a.ovf.cap);
   typedef typename Vec4T<R>::type V4;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-    const V4* src = reinterpret_cast<const V4*>(a.ovf.rec + (size_t)i * kRecordLen);
+    const V4* src = reinterpret_cast<const V4*>(a.ovf.rec + (size_t)i * RecordLen<R>::value);
     const V4 r0 = src[0], r1 = src[1], r2 = src[2];
     R pos[3] = {r0.x, r0.y, r0.z};
     R rem[3] = {r0.w, r1.x, r1.y};
