@@ -251,7 +251,10 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   p.err = c->err;
   p.movers = c->movers;
   p.max_count = c->max_count;
-  k_push_pass<R><<<blocks, kBlock, 0, c->stream>>>(p);
+  if (compact)
+    k_push_pass<R><<<blocks, kBlock, 0, c->stream>>>(p);
+  else
+    k_push_rows<R><<<blocks, kBlock, 0, c->stream>>>(p);
   CHECK_LAUNCH();
 
   // the streak after the 1st crossing, found through the departure masks

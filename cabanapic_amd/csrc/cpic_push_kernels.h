@@ -92,16 +92,19 @@ __device__ __forceinline__ int face_neighbor(const CellGrid& g, int ci, int d) {
   return ix + g.nx * (iy + g.ny * iz);
 }
 
+// Wave-wide reductions; the result is read back as a scalar, so the
+// compiler knows it is the same in every lane (branches and loops on it are
+// then scalar, not exec-masked).
 __device__ __forceinline__ int wave_max_int(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 __device__ __forceinline__ int wave_sum_int(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 // Marks a loaded vector as consumed at this point (an empty asm that reads
@@ -718,6 +721,165 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   if (valid && !a.last) {
 #pragma unroll
     for (int dd = 0; dd < 6; ++dd) a.out.cnt[(size_t)dd * NC + ci] = min(kd[dd], a.out.M);
+  }
+}
+
+// Pass 1 in place (every step but the compacting ones).  Same arithmetic as
+// k_push_pass; the memory pipeline differs.  Every lane stores to its row j
+// in every row, whether it holds a particle or not (a row without a particle
+// is free in the output column, and arrivals overwrite it later), so every
+// row issues the same loads and stores.  The loads of row j + 1 are issued
+// before the stores of row j, and with a fixed number of instructions in
+// between, the wait for them does not wait for those stores.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
+  const bool valid = ci < a.g.nc;
+  const int len = valid ? a.cs.cnt_in[ci] : 0;
+  // rows walked: the longest column of the wave, rounded up to even so that
+  // both rows of every loop iteration are processed (D is a multiple of 8,
+  // so the extra row is a row past every column's end)
+  const int lreal = wave_max_int(len);
+  const int lmax = (lreal + 1) & ~1;
+  const int v = valid ? interior_to_voxel(a.g, ci) : 0;
+  typedef typename Vec4T<R>::type V4;
+  V4 e0 = {}, e1 = {}, e2 = {}, b0 = {}, b1 = {};
+  if (valid) {
+    const V4* f = reinterpret_cast<const V4*>(a.interp + (size_t)v * kInterpStride);
+    e0 = f[0];
+    e1 = f[1];
+    e2 = f[2];
+    b0 = f[3];
+    b1 = f[4];
+  }
+  const R one = 1.;
+  const R one_third = 1. / 3.;
+  const R qdt_2mc = a.qdt_2mc;
+  R acc[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] = 0;
+  int hi = 0, nmov = 0;
+  unsigned long long m_in = 0, m_out = 0;
+  unsigned long long dw0 = 0, dw1 = 0, dw2 = 0, dw3 = 0, dw4 = 0, dw5 = 0;
+  const size_t MP = a.cs.nc_pad;
+  const int MW = a.cs.MW;
+  const R* const dummy = a.cs.in;
+
+  // loads of row r into (x, y); returns whether the row holds a particle
+  auto request = [&](int r, V4& x, V4& y) -> bool {
+    if ((r & 63) == 0) {
+      m_in = (valid && r < len) ? a.cs.mask_in[(size_t)(r >> 6) * MP + ci] : 0ull;
+      settle(m_in);  // once per 64 rows
+    }
+    const bool ok = valid && r < len && ((m_in >> (r & 63)) & 1ull);
+    const V4* src = reinterpret_cast<const V4*>(ok ? a.cs.in + col_slot(ci, r, a.cs.D) * kSlotLen
+                                                   : dummy);
+    x = src[0];
+    y = src[1];
+    return ok;
+  };
+
+  auto process = [&](int j, bool have, const V4& pa, const V4& pb) {
+    const int b = j & 63;
+    V4 o0 = pa, o1 = pb;  // what row j of the output receives
+    if (have) {
+      const R dx = pa.x, dy = pa.y, dz = pa.z;
+      R ux = pa.w, uy = pb.x, uz = pb.y;
+      const R w = pb.z;
+      boris(e0, e1, e2, b0, b1, qdt_2mc, dx, dy, dz, ux, uy, uz);
+      R ddx, ddy, ddz;
+      displacement(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, ddx, ddy, ddz);
+      const R mx = dx + ddx, my = dy + ddy, mz = dz + ddz;
+      const R nx = mx + ddx, ny = my + ddy, nz = mz + ddz;
+      const R q = w * a.qsp;
+      R px, py, pz;
+      if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
+        const R v5 = q * ddx * ddy * ddz * one_third;
+        R o[12];
+        streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc[k] += o[k];
+        px = nx;
+        py = ny;
+        pz = nz;
+      } else {
+        ++nmov;
+        R pos[3] = {dx, dy, dz};
+        R rem[3] = {ddx, ddy, ddz};
+        const int d = streak(pos, rem, q, acc);
+        // a particle that leaves is stored with its position before the push
+        px = d < 0 ? pos[0] : dx;
+        py = d < 0 ? pos[1] : dy;
+        pz = d < 0 ? pos[2] : dz;
+        const unsigned long long bit = 1ull << b;
+        dw0 |= d == 0 ? bit : 0ull;
+        dw1 |= d == 1 ? bit : 0ull;
+        dw2 |= d == 2 ? bit : 0ull;
+        dw3 |= d == 3 ? bit : 0ull;
+        dw4 |= d == 4 ? bit : 0ull;
+        dw5 |= d == 5 ? bit : 0ull;
+      }
+      m_out |= 1ull << b;
+      hi = j + 1;
+      o0.x = px; o0.y = py; o0.z = pz; o0.w = ux;
+      o1.x = uy; o1.y = uz; o1.z = w;  o1.w = 0;
+    }
+    V4* dst = reinterpret_cast<V4*>(a.cs.out + col_slot(ci, j, a.cs.D) * kSlotLen);
+    dst[0] = o0;
+    dst[1] = o1;
+    if (b == 63 || j == lmax - 1) {
+      const int wd = j >> 6;
+      a.cs.mask_out[(size_t)wd * MP + ci] = m_out;
+      a.dep[dep_index(0, wd, MW, MP, ci)] = dw0;
+      a.dep[dep_index(1, wd, MW, MP, ci)] = dw1;
+      a.dep[dep_index(2, wd, MW, MP, ci)] = dw2;
+      a.dep[dep_index(3, wd, MW, MP, ci)] = dw3;
+      a.dep[dep_index(4, wd, MW, MP, ci)] = dw4;
+      a.dep[dep_index(5, wd, MW, MP, ci)] = dw5;
+      m_out = dw0 = dw1 = dw2 = dw3 = dw4 = dw5 = 0;
+    }
+  };
+
+  // two row buffers in turn, one row ahead.  Each wait is preceded by the
+  // same instructions on every path into it (two loads and two stores after
+  // the awaited loads), so it waits for the awaited loads only; on entry,
+  // two stores of values rewritten at the end stand in for the stores of
+  // the previous row.
+  V4 xa = {}, ya = {}, xb = {}, yb = {};
+  bool ha = request(0, xa, ya);
+  bool hb = request(1, xb, yb);
+  a.cs.cnt_out[ci] = 0;
+  a.cs.mask_out[ci] = 0ull;
+  for (int j = 0; j < lmax; j += 2) {
+    settle(xa);
+    settle(ya);
+    process(j, ha, xa, ya);
+    ha = request(j + 2, xa, ya);
+    settle(xb);
+    settle(yb);
+    process(j + 1, hb, xb, yb);
+    hb = request(j + 3, xb, yb);
+  }
+
+  if (valid) {
+    const size_t N = a.ncells;
+    if (a.accumulate) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc[k];
+    }
+  }
+  for (int w = (lmax + 63) >> 6; w < MW; ++w) {
+    a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
+    for (int d = 0; d < 6; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
+  }
+  a.cs.cnt_out[ci] = hi;
+  const int wm = wave_sum_int(nmov);
+  if ((threadIdx.x & 63) == 0) {
+    if (wm) atomicAdd(a.movers, (unsigned)wm);
+    atomicMax(a.max_count, lreal);
   }
 }
 
