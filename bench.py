@@ -200,7 +200,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(push_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(push_gbs / HBM_PEAK_GBS, 4),
-                     "kernel": "push stage: k_push_pass + 3 x k_finish_pass + k_spill_pass",
+                     "kernel": "push stage: k_push_rows (k_push_pass every 8th step) + k_arrive_pass "
+                               "+ 2 x k_finish_pass + k_spill_pass",
                      "traffic": measured_push_traffic(args.n, args.nppc),
                      "traffic_unit": "bytes per push stage (one step)",
                      "algorithmic_bytes": bp},
