@@ -65,6 +65,7 @@ struct cpic_context {
   unsigned long long* mask[2] = {};  // occupied rows, [MW][nc_pad]
   int MW = 0;              // mask words per cell
   int cur = 0;             // buffer holding the current particles
+  long pushes = 0;         // cpic_push calls since the last upload (every 8th compacts)
   // crossing records: list 1 (first and third crossings), list 2 (second)
   int M1 = 0, M2 = 0;
   void* rec1 = nullptr;
@@ -633,6 +634,7 @@ int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void*
   HIP_TRY(hipMemcpy(c->mask[0], mask.data(), sizeof(unsigned long long) * mask.size(),
                     hipMemcpyHostToDevice));
   c->cur = 0;
+  c->pushes = 0;
   c->np = n;
   return CPIC_OK;
 }
@@ -781,7 +783,9 @@ int cpic_push(cpic_context* c, double qdt_2mc, double cdt_dx, double cdt_dy, dou
   if (int s = check_periodic(boundary)) return s;
   return with_real(c, [&](auto r) {
     using R = decltype(r);
-    return stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1, 1);
+    // the first push after an upload and every 8th one pack the columns
+    const int compact = (c->pushes++ % 8 == 0) ? 1 : 0;
+    return stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1, compact);
   });
 }
 

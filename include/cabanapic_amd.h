@@ -112,7 +112,9 @@ int cpic_clear_accumulator_array(cpic_context *ctx);
 
 /* push<ScatterView> with the inline move_p, src/push.h:7-300,
  * src/move_p.h:59-374: Boris push, cell crossing and charge-conserving
- * deposit into the accumulators in one kernel.  boundary must be
+ * deposit, added to the accumulators as the reference's push does.  The
+ * first push after an upload and every 8th one also pack the particle
+ * columns (no visible effect but the order within a cell).  boundary must be
  * CPIC_BOUNDARY_PERIODIC. */
 int cpic_push(cpic_context *ctx, double qdt_2mc, double cdt_dx,
               double cdt_dy, double cdt_dz, double qsp, int32_t boundary);
