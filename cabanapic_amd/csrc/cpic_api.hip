@@ -252,7 +252,11 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   p.err = c->err;
   p.movers = c->movers;
   p.max_count = c->max_count;
-  if (compact)
+  // k_push_rows stores a full row per lane and row (dummy writes included)
+  // and needs more registers; measured faster in float, slower in double
+  // (64-byte slots, 2 waves per SIMD), where k_push_pass keeps the rows in
+  // place too
+  if (compact || sizeof(R) == 8)
     k_push_pass<R><<<blocks, kBlock, 0, c->stream>>>(p);
   else
     k_push_rows<R><<<blocks, kBlock, 0, c->stream>>>(p);
