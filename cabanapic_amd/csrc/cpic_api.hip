@@ -77,7 +77,8 @@ struct cpic_context {
   unsigned int* ovf_count = nullptr;
   int ovf_cap = 0;
   int* max_count = nullptr;  // largest column occupancy seen by the last push
-  unsigned long long* dep = nullptr;  // departure masks of pass 1, [6][MW][nc_pad]
+  unsigned long long* dep = nullptr;  // departure masks of pass 1, [7][MW][nc_pad]
+  int* dep_words = nullptr;           // mask words written by pass 1, per group
 
   void* fields = nullptr;
   void* interp = nullptr;
@@ -238,6 +239,7 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   PushPass<R> p;
   p.cs = cs;
   p.dep = c->dep;
+  p.dep_words = c->dep_words;
   p.interp = (const R*)c->interp;
   p.acc = (R*)c->acc;
   p.g = c->cg;
@@ -266,6 +268,7 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   ArrivePass<R> ar;
   ar.cs = cs;
   ar.dep = c->dep;
+  ar.dep_words = c->dep_words;
   ar.out = L2;
   ar.ovf = ovf;
   ar.acc = (R*)c->acc;
@@ -413,7 +416,7 @@ int ensure_columns(cpic_context* c, int D, int M1, int M2) {
     c->mask[b] = nullptr;
   }
   void** recs[] = {&c->rec1, &c->rec2, (void**)&c->rc1, (void**)&c->rc2, &c->ovf_rec,
-                   (void**)&c->ovf_cell, (void**)&c->dep};
+                   (void**)&c->ovf_cell, (void**)&c->dep, (void**)&c->dep_words};
   for (void** p : recs) {
     if (*p) HIP_TRY(hipFree(*p));
     *p = nullptr;
@@ -438,9 +441,12 @@ int ensure_columns(cpic_context* c, int D, int M1, int M2) {
   if ((s = fill_zero<int>(c, c->cnt[0], c->nc_pad))) return s;
   if ((s = fill_zero<int>(c, c->cnt[1], c->nc_pad))) return s;
   const int MW = (D + 63) / 64;
-  if ((s = alloc((void**)&c->dep, sizeof(unsigned long long) * 6 * (size_t)MW * c->nc_pad)))
+  if ((s = alloc((void**)&c->dep,
+                 sizeof(unsigned long long) * kDepMasks * (size_t)MW * c->nc_pad)))
     return s;
-  if ((s = fill_zero<unsigned long long>(c, c->dep, 6 * (size_t)MW * c->nc_pad))) return s;
+  if ((s = fill_zero<unsigned long long>(c, c->dep, kDepMasks * (size_t)MW * c->nc_pad))) return s;
+  if ((s = alloc((void**)&c->dep_words, sizeof(int) * (c->nc_pad / 64)))) return s;
+  if ((s = fill_zero<int>(c, c->dep_words, c->nc_pad / 64))) return s;
   if ((s = fill_zero<unsigned long long>(c, c->mask[0], (size_t)MW * c->nc_pad))) return s;
   if ((s = fill_zero<unsigned long long>(c, c->mask[1], (size_t)MW * c->nc_pad))) return s;
   c->D = D;
@@ -564,7 +570,7 @@ int cpic_destroy(cpic_context* c) {
   void* ptrs[] = {c->rec1,   c->rec2,     c->rc1,      c->rc2,     c->ovf_rec,
                   c->ovf_cell, c->ovf_count, c->max_count, c->fields, c->interp,
                   c->acc,    c->partials, c->red_out,  c->err,     c->movers,
-                  c->hist_dev, c->dep};
+                  c->hist_dev, c->dep, c->dep_words};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);

@@ -397,7 +397,10 @@ constexpr int kRowBatch = 2;  // rows loaded together by pass 1 (divides 64)
 
 // Departure masks: bit r of word w of (face d, cell c) is set when the
 // particle stored in row 64w + r of cell c's output column left the cell
-// through face d in pass 1.  Layout [d][w][nc_pad].
+// through face d in pass 1.  Layout [d][w][nc_pad]; entry d = 6 holds the
+// union of the six (the cell's own departures), and dep_words[group] the
+// number of words of the group's columns that pass 1 wrote.
+constexpr int kDepMasks = 7;
 __device__ __forceinline__ size_t dep_index(int d, int w, int MW, size_t MP, int ci) {
   return ((size_t)d * (size_t)MW + (size_t)w) * MP + (size_t)ci;
 }
@@ -405,6 +408,7 @@ __device__ __forceinline__ size_t dep_index(int d, int w, int MW, size_t MP, int
 template <class R> struct PushPass {
   ColState<R> cs;
   unsigned long long* dep;  // departure masks written by this pass
+  int* dep_words;           // [nc_pad / 64] words written per group
   const R* interp;
   R* acc;  // slot-major [12][Ng]
   CellGrid g;
@@ -462,6 +466,7 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
     a.dep[dep_index(3, cw, MW, MP, ci)] = dw3;
     a.dep[dep_index(4, cw, MW, MP, ci)] = dw4;
     a.dep[dep_index(5, cw, MW, MP, ci)] = dw5;
+    a.dep[dep_index(6, cw, MW, MP, ci)] = dw0 | dw1 | dw2 | dw3 | dw4 | dw5;
     dw0 = dw1 = dw2 = dw3 = dw4 = dw5 = 0;
   };
 
@@ -580,12 +585,12 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
     flush_dep();
     for (int w = 0; w < MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = low_bits(ns - 64 * w);
     for (int w = cw + 1; w < MW; ++w)
-      for (int d = 0; d < 6; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
+      for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
     a.cs.cnt_out[ci] = ns;
   } else {
     for (int w = (lmax + 63) >> 6; w < MW; ++w) {
       a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
-      for (int d = 0; d < 6; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
+      for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
     }
     a.cs.cnt_out[ci] = hi;
   }
@@ -593,6 +598,7 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   if ((threadIdx.x & 63) == 0) {
     if (wm) atomicAdd(a.movers, (unsigned)wm);
     atomicMax(a.max_count, lmax);
+    a.dep_words[ci >> 6] = (lmax + 63) >> 6;
   }
 }
 
@@ -836,6 +842,7 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
       a.dep[dep_index(3, wd, MW, MP, ci)] = dw3;
       a.dep[dep_index(4, wd, MW, MP, ci)] = dw4;
       a.dep[dep_index(5, wd, MW, MP, ci)] = dw5;
+      a.dep[dep_index(6, wd, MW, MP, ci)] = dw0 | dw1 | dw2 | dw3 | dw4 | dw5;
       m_out = dw0 = dw1 = dw2 = dw3 = dw4 = dw5 = 0;
     }
   };
@@ -873,19 +880,21 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
   }
   for (int w = (lmax + 63) >> 6; w < MW; ++w) {
     a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
-    for (int d = 0; d < 6; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
+    for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
   }
   a.cs.cnt_out[ci] = hi;
   const int wm = wave_sum_int(nmov);
   if ((threadIdx.x & 63) == 0) {
     if (wm) atomicAdd(a.movers, (unsigned)wm);
     atomicMax(a.max_count, lreal);
+    a.dep_words[ci >> 6] = (lmax + 63) >> 6;
   }
 }
 
 template <class R> struct ArrivePass {
   ColState<R> cs;                        // cs.out: the columns of pass 1
   const unsigned long long* dep;         // departure masks of pass 1
+  const int* dep_words;                  // words of them written per group
   RecordList<R> out;                     // records of the next crossings
   Overflow<R> ovf;
   R* acc;
@@ -935,7 +944,16 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
   int total_all = 0;
   const R* dummy = a.cs.out;
 
-  for (int w = 0; w < MW; ++w) {
+  // words of the departure masks to read: those pass 1 wrote for the
+  // source groups (usually one)
+  int nw = 0;
+  if (valid) {
+    nw = max(max(a.dep_words[s0 >> 6], a.dep_words[s1 >> 6]),
+             max(max(a.dep_words[s2 >> 6], a.dep_words[s3 >> 6]),
+                 max(a.dep_words[s4 >> 6], a.dep_words[s5 >> 6])));
+  }
+  nw = min(wave_max_int(nw), MW);
+  for (int w = 0; w < nw; ++w) {
     unsigned long long d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0;
     if (valid) {
       d0 = a.dep[dep_index(0, w, MW, MP, s0)];
@@ -1024,10 +1042,9 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
     // are cleared here.  Other lanes may still be reading those rows in this
     // pass, but no lane writes them before the next pass, which is the first
     // that can hand them out again.
-    for (int w = 0; w < MW; ++w) {
-      unsigned long long own = 0;
-#pragma unroll
-      for (int dd = 0; dd < 6; ++dd) own |= a.dep[dep_index(dd, w, MW, MP, ci)];
+    const int own_words = min(a.dep_words[ci >> 6], MW);
+    for (int w = 0; w < own_words; ++w) {
+      const unsigned long long own = a.dep[dep_index(6, w, MW, MP, ci)];
       if (own) a.cs.mask_out[(size_t)w * MP + ci] &= ~own;
     }
 #pragma unroll
