@@ -125,8 +125,8 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): this build has no "
-              "multi-GPU path; see DESIGN.md for how the step would shard.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the step has no "
+              "multi-GPU path; its domain decomposition is a design, not code.", file=sys.stderr)
         sys.exit(2)
 
     from cabanapic_amd import decks
@@ -200,8 +200,8 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(push_gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(push_gbs / HBM_PEAK_GBS, 4),
-                     "kernel": "push stage: k_push_rows (k_push_pass every 8th step) + k_arrive_pass "
-                               "+ 2 x k_finish_pass + k_spill_pass",
+                     "kernel": "push stage: k_push_rows (k_push_pass every 32nd step) + "
+                               "k_arrive_pass + 2 x k_finish_pass + k_spill_pass",
                      "traffic": measured_push_traffic(args.n, args.nppc),
                      "traffic_unit": "bytes per push stage (one step)",
                      "algorithmic_bytes": bp},

@@ -63,6 +63,29 @@ __device__ __forceinline__ size_t col_slot(int ci, int row, int D) {
   return ((size_t)(ci >> 6) * (size_t)D + (size_t)row) * 64 + (size_t)(ci & 63);
 }
 
+// Third word of a gfx9 buffer descriptor for raw dword access (32-bit data
+// format; no swizzle, no index stride).
+constexpr int kBufferWord3 = 0x00020000;
+typedef unsigned U32x4 __attribute__((ext_vector_type(4)));
+
+// One slot (two 4-wide vectors) stored through buffer descriptor rs at byte
+// offset off; an offset at or past the descriptor's size drops the store.
+__device__ __forceinline__ void store_slot(__amdgpu_buffer_rsrc_t rs, int off,
+                                           const Vec4T<float>::type& a,
+                                           const Vec4T<float>::type& b) {
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U32x4, a), rs, off, 0, 0);
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U32x4, b), rs, off + 16, 0, 0);
+}
+__device__ __forceinline__ void store_slot(__amdgpu_buffer_rsrc_t rs, int off,
+                                           const Vec4T<double>::type& a,
+                                           const Vec4T<double>::type& b) {
+  typedef double D2 __attribute__((ext_vector_type(2)));
+  const D2 p[4] = {a.xy, a.zw, b.xy, b.zw};
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U32x4, p[k]), rs, off + 16 * k, 0, 0);
+}
+
 struct CellGrid {
   int nx, ny, nz;  // interior extents
   int nc;          // nx*ny*nz
@@ -415,7 +438,6 @@ template <class R> struct PushPass {
   int ncells;  // Ng
   R qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
   int accumulate;  // 1: acc += (push semantics), 0: acc = (after a clear)
-  int compact;     // 1: pack the rows to 0,1,..; 0: keep every particle in its row
   int* err;
   unsigned* movers;
   int* max_count;
@@ -452,8 +474,8 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   R acc[12];
 #pragma unroll
   for (int k = 0; k < 12; ++k) acc[k] = 0;
-  int ns = 0, hi = 0, nmov = 0;
-  unsigned long long m_in = 0, m_out = 0;
+  int ns = 0, nmov = 0;
+  unsigned long long m_in = 0;
   const size_t MP = a.cs.nc_pad;
   const int MW = a.cs.MW;
   // departure bits of output word cw, one word per face
@@ -470,8 +492,7 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
     dw0 = dw1 = dw2 = dw3 = dw4 = dw5 = 0;
   };
 
-  auto process = [&](int j, bool have, const V4& pa, const V4& pb) {
-    const int b = j & 63;
+  auto process = [&](bool have, const V4& pa, const V4& pb) {
     if (have) {
       const R dx = pa.x, dy = pa.y, dz = pa.z;
       R ux = pa.w, uy = pb.x, uz = pb.y;
@@ -504,17 +525,10 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
         py = d < 0 ? pos[1] : dy;
         pz = d < 0 ? pos[2] : dz;
       }
-      int row;
-      if (a.compact) {
-        row = ns++;
-        if ((row >> 6) != cw) {
-          flush_dep();
-          cw = row >> 6;
-        }
-      } else {
-        row = j;
-        m_out |= 1ull << b;
-        hi = j + 1;
+      const int row = ns++;
+      if ((row >> 6) != cw) {
+        flush_dep();
+        cw = row >> 6;
       }
       if (d >= 0) {
         const unsigned long long bit = 1ull << (row & 63);
@@ -526,12 +540,6 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
         dw5 |= d == 5 ? bit : 0ull;
       }
       store_particle(a.cs, ci, row, px, py, pz, ux, uy, uz, w);
-    }
-    if (!a.compact && (b == 63 || j == lmax - 1)) {
-      a.cs.mask_out[(size_t)(j >> 6) * MP + ci] = m_out;
-      m_out = 0;
-      flush_dep();
-      cw = (j >> 6) + 1;
     }
   };
 
@@ -567,7 +575,7 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
     }
 #pragma unroll
     for (int u = 0; u < kRowBatch; ++u)
-      if (j0 + u < lmax) process(j0 + u, have[u], pa[u], pb[u]);
+      if (j0 + u < lmax) process(have[u], pa[u], pb[u]);
   }
 
   const size_t N = a.ncells;
@@ -581,19 +589,11 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
     }
   }
   // the remaining mask and departure words of the column
-  if (a.compact) {
-    flush_dep();
-    for (int w = 0; w < MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = low_bits(ns - 64 * w);
-    for (int w = cw + 1; w < MW; ++w)
-      for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
-    a.cs.cnt_out[ci] = ns;
-  } else {
-    for (int w = (lmax + 63) >> 6; w < MW; ++w) {
-      a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
-      for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
-    }
-    a.cs.cnt_out[ci] = hi;
-  }
+  flush_dep();
+  for (int w = 0; w < MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = low_bits(ns - 64 * w);
+  for (int w = cw + 1; w < MW; ++w)
+    for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
+  a.cs.cnt_out[ci] = ns;
   const int wm = wave_sum_int(nmov);
   if ((threadIdx.x & 63) == 0) {
     if (wm) atomicAdd(a.movers, (unsigned)wm);
@@ -731,12 +731,14 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
 }
 
 // Pass 1 in place (every step but the compacting ones).  Same arithmetic as
-// k_push_pass; the memory pipeline differs.  Every lane stores to its row j
-// in every row, whether it holds a particle or not (a row without a particle
-// is free in the output column, and arrivals overwrite it later), so every
-// row issues the same loads and stores.  The loads of row j + 1 are issued
-// before the stores of row j, and with a fixed number of instructions in
-// between, the wait for them does not wait for those stores.
+// k_push_pass; the memory pipeline differs.  Every row issues the same loads
+// and stores on every lane: a lane without a particle in the row stores
+// whatever it loaded when another lane of its 128-byte line holds one (the
+// row is free in the output column, and arrivals overwrite it later), and
+// otherwise its store is dropped by the buffer descriptor's range check.
+// The loads of row j + 1 are issued before the stores of row j, and with a
+// fixed number of instructions in between, the wait for them does not wait
+// for those stores.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
@@ -770,6 +772,20 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
   const size_t MP = a.cs.nc_pad;
   const int MW = a.cs.MW;
   const R* const dummy = a.cs.in;
+  // Row stores go through a buffer descriptor over the wave's column group:
+  // a store at an offset past its end is dropped without a branch.  A lane
+  // stores whenever a lane of its 128-byte line of the row holds a particle
+  // (a free row receives whatever was loaded), so memory only ever sees whole
+  // lines (a partly written line costs a read-modify-write), and rows past
+  // the ends of all the columns of a line cost nothing.
+  const int lane = threadIdx.x & 63;
+  const int grp = __builtin_amdgcn_readfirstlane(ci >> 6);
+  const int gbytes = a.cs.D * 64 * kSlotLen * (int)sizeof(R);
+  const __amdgpu_buffer_rsrc_t rows_out = __builtin_amdgcn_make_buffer_rsrc(
+      a.cs.out + (size_t)grp * a.cs.D * 64 * kSlotLen, 0, gbytes, kBufferWord3);
+  constexpr int kLine = 128 / (kSlotLen * (int)sizeof(R));  // lanes per line
+  static_assert(kLine >= 1 && kLine < 64 && (kLine & (kLine - 1)) == 0, "line of slots");
+  const int line0 = lane & ~(kLine - 1);
 
   // loads of row r into (x, y); returns whether the row holds a particle
   auto request = [&](int r, V4& x, V4& y) -> bool {
@@ -830,9 +846,10 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
       o0.x = px; o0.y = py; o0.z = pz; o0.w = ux;
       o1.x = uy; o1.y = uz; o1.z = w;  o1.w = 0;
     }
-    V4* dst = reinterpret_cast<V4*>(a.cs.out + col_slot(ci, j, a.cs.D) * kSlotLen);
-    dst[0] = o0;
-    dst[1] = o1;
+    const unsigned long long occ = __ballot(have);
+    const bool st = (occ >> line0) & ((1ull << kLine) - 1);
+    const int off = st ? (j * 64 + lane) * kSlotLen * (int)sizeof(R) : gbytes;
+    store_slot(rows_out, off, o0, o1);
     if (b == 63 || j == lmax - 1) {
       const int wd = j >> 6;
       a.cs.mask_out[(size_t)wd * MP + ci] = m_out;

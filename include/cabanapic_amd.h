@@ -63,9 +63,9 @@ typedef struct cpic_step_params {
   int32_t energy_interval;  /* E,B energies every k steps (0 = never);
                                the reference dumps every step
                                (ENERGY_DUMP_INTERVAL, example.cpp:53-55)   */
-  int32_t compact_interval; /* particles stay binned by cell; every k steps
-                               the push also packs each cell's column
-                               (0 = every 8 steps)                           */
+  int32_t compact_interval; /* particles stay binned by cell; every k-th
+                               push also packs each cell's column
+                               (0 = every 32nd push)                         */
   int32_t kinetic_interval; /* kinetic energy every k steps (0 = never)     */
 } cpic_step_params;
 
@@ -113,7 +113,7 @@ int cpic_clear_accumulator_array(cpic_context *ctx);
 /* push<ScatterView> with the inline move_p, src/push.h:7-300,
  * src/move_p.h:59-374: Boris push, cell crossing and charge-conserving
  * deposit, added to the accumulators as the reference's push does.  The
- * first push after an upload and every 8th one also pack the particle
+ * first push after an upload and every 32nd one also pack the particle
  * columns (no visible effect but the order within a cell).  boundary must be
  * CPIC_BOUNDARY_PERIODIC. */
 int cpic_push(cpic_context *ctx, double qdt_2mc, double cdt_dx,
