@@ -6,4 +6,4 @@ C-ABI declared in ``include/cabanapic_amd.h``.
 """
 from ._capi import CpicError, load as load_library  # noqa: F401
 from .pic import (FIELD_MEMBERS, INTERP_MEMBERS, PARTICLE_MEMBERS, Simulation,  # noqa: F401
-                  make_step_params)
+                  TangentSimulation, make_step_params)

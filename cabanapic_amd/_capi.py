@@ -85,6 +85,14 @@ SIGNATURES = {
     "cpic_last_push_movers": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "cpic_device_errors": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
     "cpic_init_plasma": (ctypes.c_int, [ctypes.POINTER(PlasmaSpec), _I32, _I64] + [_P] * 8),
+    "cpic_tangent_create": (ctypes.c_int, [ctypes.POINTER(Config), _PP]),
+    "cpic_tangent_destroy": (ctypes.c_int, [_P]),
+    "cpic_tangent_contexts": (ctypes.c_int, [_P, _PP, _PP]),
+    "cpic_tangent_upload_particles": (ctypes.c_int, [_P, _I64] + [_P] * 9),
+    "cpic_tangent_download_particles": (ctypes.c_int, [_P, _I64] + [_P] * 9),
+    "cpic_tangent_push": (ctypes.c_int, [_P, _D, _D, _D, _D, _D, _I32]),
+    "cpic_tangent_step": (ctypes.c_int, [_P, ctypes.POINTER(StepParams), _I64, _P]),
+    "cpic_tangent_last_push_movers": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
 }
 
 

@@ -16,6 +16,7 @@
 
 #include "../../include/cabanapic_amd.h"
 #include "cpic_push_kernels.h"
+#include "cpic_tangent_kernels.h"
 
 using namespace cpic;
 
@@ -979,3 +980,5 @@ int cpic_device_errors(cpic_context* c, int32_t* flags) {
 }
 
 }  // extern "C"
+
+#include "cpic_tangent.inc"

@@ -1,0 +1,220 @@
+// cpic_tangent_kernels.h -- forward-mode (tangent-linear) derivative of the
+// push + move_p + deposit.
+//
+// The reference has no derivative of its step.  The tangent of the step
+// with respect to one deck parameter theta is carried alongside the primal
+// state: every particle carries d(x,y,z,ux,uy,uz)/dtheta, and a second set
+// of field arrays carries dE/dtheta, dB/dtheta and dJ/dtheta.  Every stage
+// but the push is linear in the fields (interpolator load, unload, ghost
+// fold and copy, advance_b, advance_e), so its tangent is the same stage
+// applied to the tangent arrays.  The push is not: its tangent is computed
+// here by running the push's own device functions (boris, displacement,
+// streak) on dual numbers, whose value part repeats the primal arithmetic
+// operation for operation (so the primal results are those of the column
+// push, bit for bit) and whose tangent part, in FP64, is the derivative of
+// each operation.  The derivative is the one of the piece of the step the
+// particle is on: crossing decisions (which face, how many streaks) are
+// taken on the values and held fixed, and a position clamped to a face has
+// zero tangent.
+#pragma once
+
+#include "cpic_push_kernels.h"
+
+namespace cpic {
+
+// A value of type R and its derivative, in FP64.  Operators are hidden
+// friends, so integer and floating constants convert implicitly.
+template <class R> struct Dual {
+  R v;
+  double t;
+  __device__ Dual() : v(0), t(0) {}
+  __device__ Dual(double c) : v((R)c), t(0) {}
+  __device__ Dual(R value, double tangent) : v(value), t(tangent) {}
+
+  friend __device__ __forceinline__ Dual operator+(const Dual& a, const Dual& b) {
+    return Dual(a.v + b.v, a.t + b.t);
+  }
+  friend __device__ __forceinline__ Dual operator-(const Dual& a, const Dual& b) {
+    return Dual(a.v - b.v, a.t - b.t);
+  }
+  friend __device__ __forceinline__ Dual operator-(const Dual& a) { return Dual(-a.v, -a.t); }
+  friend __device__ __forceinline__ Dual operator*(const Dual& a, const Dual& b) {
+    return Dual(a.v * b.v, a.t * (double)b.v + (double)a.v * b.t);
+  }
+  friend __device__ __forceinline__ Dual operator/(const Dual& a, const Dual& b) {
+    const R q = a.v / b.v;
+    return Dual(q, (a.t - (double)q * b.t) / (double)b.v);
+  }
+  __device__ __forceinline__ Dual& operator+=(const Dual& b) { return *this = *this + b; }
+  __device__ __forceinline__ Dual& operator-=(const Dual& b) { return *this = *this - b; }
+  __device__ __forceinline__ Dual& operator*=(const Dual& b) { return *this = *this * b; }
+  friend __device__ __forceinline__ bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+  friend __device__ __forceinline__ bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+  friend __device__ __forceinline__ bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+  friend __device__ __forceinline__ bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+};
+
+// the narrowed square root of the push and its derivative 1 / (2 sqrt)
+template <class R> __device__ __forceinline__ Dual<R> sqrtf_narrow(const Dual<R>& s) {
+  const R r = sqrtf_narrow(s.v);
+  return Dual<R>(r, s.t * 0.5 / (double)r);
+}
+
+// the streak correction term q a b c / 3 and its derivative
+template <class R>
+__device__ __forceinline__ Dual<R> streak_v5(const Dual<R>& q, const Dual<R>& a,
+                                             const Dual<R>& b, const Dual<R>& c) {
+  const double Q = q.v, A = a.v, B = b.v, C = c.v;
+  const double t = (q.t * A * B * C + Q * a.t * B * C + Q * A * b.t * C + Q * A * B * c.t) * (1. / 3.);
+  return Dual<R>(streak_v5(q.v, a.v, b.v, c.v), t);
+}
+
+template <class R> struct DualV4 {
+  Dual<R> x, y, z, w;
+};
+
+template <class R> struct TangentPush {
+  R* part;           // [7][stride]: x, y, z, ux, uy, uz, w
+  int* cell;         // voxel of each particle
+  double* tpart;     // [6][stride]: tangents of x, y, z, ux, uy, uz
+  long stride;
+  int n;
+  const R* interp;          // primal interpolators
+  const double* tinterp;    // tangent interpolators
+  R* acc;                   // primal accumulators, slot-major [12][Ng]
+  double* tacc;             // tangent accumulators
+  CellGrid g;
+  int ncells;               // Ng
+  R qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
+  int* err;
+  unsigned* movers;
+};
+
+template <class R>
+__device__ __forceinline__ void deposit_dual(const TangentPush<R>& a, int v, const Dual<R> o[12]) {
+  const size_t N = a.ncells;
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    unsafeAtomicAdd(a.acc + k * N + v, o[k].v);
+    unsafeAtomicAdd(a.tacc + k * N + v, o[k].t);
+  }
+}
+
+// One thread per particle: the push of src/push.h:65-295 with the move_p
+// loop of src/move_p.h:93-371, on dual numbers, deposits added atomically
+// (primal to acc, tangent to tacc).  The primal positions, momenta and cells
+// equal those of the column push; the primal accumulators equal them up to
+// the order of the floating-point sums.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= a.n) return;
+  typedef Dual<R> D;
+  typedef typename Vec4T<R>::type V4;
+  typedef double T4 __attribute__((ext_vector_type(4)));
+  const long S = a.stride;
+  const int v0 = a.cell[p];
+  const int X = a.g.sy, Y = a.g.sz / a.g.sy;
+  const int ix = v0 % X - 1, iy = (v0 / X) % Y - 1, iz = v0 / a.g.sz - 1;
+  if (v0 < 0 || v0 >= a.ncells || ix < 0 || ix >= a.g.nx || iy < 0 || iy >= a.g.ny || iz < 0 ||
+      iz >= a.g.nz) {
+    atomicOr(a.err, kErrBadCell);
+    return;
+  }
+  int ci = ix + a.g.nx * (iy + a.g.ny * iz);
+
+  const V4* f = reinterpret_cast<const V4*>(a.interp + (size_t)v0 * kInterpStride);
+  const T4* tf = reinterpret_cast<const T4*>(a.tinterp + (size_t)v0 * kInterpStride);
+  DualV4<R> e[5];
+#pragma unroll
+  for (int k = 0; k < 5; ++k) {
+    const V4 fv = f[k];
+    const T4 tv = tf[k];
+    e[k].x = D(fv.x, tv.x);
+    e[k].y = D(fv.y, tv.y);
+    e[k].z = D(fv.z, tv.z);
+    e[k].w = D(fv.w, tv.w);
+  }
+  const D dx(a.part[p], a.tpart[p]);
+  const D dy(a.part[S + p], a.tpart[S + p]);
+  const D dz(a.part[2 * S + p], a.tpart[2 * S + p]);
+  D ux(a.part[3 * S + p], a.tpart[3 * S + p]);
+  D uy(a.part[4 * S + p], a.tpart[4 * S + p]);
+  D uz(a.part[5 * S + p], a.tpart[5 * S + p]);
+  const R w = a.part[6 * S + p];
+
+  boris(e[0], e[1], e[2], e[3], e[4], D(a.qdt_2mc, 0.), dx, dy, dz, ux, uy, uz);
+  D ddx, ddy, ddz;
+  displacement(ux, uy, uz, D(a.cdt_dx, 0.), D(a.cdt_dy, 0.), D(a.cdt_dz, 0.), ddx, ddy, ddz);
+  const D one = 1.;
+  const D one_third = 1. / 3.;
+  const D mx = dx + ddx, my = dy + ddy, mz = dz + ddz;
+  const D nx = mx + ddx, ny = my + ddy, nz = mz + ddz;
+  const D q = D(w, 0.) * D(a.qsp, 0.);
+  D pos[3];
+  if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
+    const D v5 = q * ddx * ddy * ddz * one_third;
+    D o[12];
+    streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
+    deposit_dual(a, v0, o);
+    pos[0] = nx;
+    pos[1] = ny;
+    pos[2] = nz;
+  } else {
+    atomicAdd(a.movers, 1u);
+    pos[0] = dx;
+    pos[1] = dy;
+    pos[2] = dz;
+    D rem[3] = {ddx, ddy, ddz};
+    bool done = false;
+    for (int it = 0; it < kMaxStreaks; ++it) {
+      D o[12];
+      const int d = streak(pos, rem, q, o);
+      deposit_dual(a, interior_to_voxel(a.g, ci), o);
+      if (d < 0) {
+        done = true;
+        break;
+      }
+      ci = face_neighbor(a.g, ci, d);
+    }
+    if (!done) atomicOr(a.err, kErrStreakLimit);
+  }
+  a.part[p] = pos[0].v;
+  a.part[S + p] = pos[1].v;
+  a.part[2 * S + p] = pos[2].v;
+  a.part[3 * S + p] = ux.v;
+  a.part[4 * S + p] = uy.v;
+  a.part[5 * S + p] = uz.v;
+  a.tpart[p] = pos[0].t;
+  a.tpart[S + p] = pos[1].t;
+  a.tpart[2 * S + p] = pos[2].t;
+  a.tpart[3 * S + p] = ux.t;
+  a.tpart[4 * S + p] = uy.t;
+  a.tpart[5 * S + p] = uz.t;
+  a.cell[p] = interior_to_voxel(a.g, ci);
+}
+
+// sum over interior cells of E . tE and B . tB -> partials[block][2] (FP64):
+// the derivatives of the field energies 0.5 sum E^2 and 0.5 sum B^2
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_field_energy_tangent(const R* __restrict__ fields, const double* __restrict__ tfields, Grid g,
+                       double* __restrict__ partials) {
+  const int nint = g.nx * g.ny * g.nz;
+  const size_t N = g.ncells;
+  double v[2] = {0, 0};
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nint; t += gridDim.x * blockDim.x) {
+    const int x = 1 + t % g.nx;
+    const int y = 1 + (t / g.nx) % g.ny;
+    const int z = 1 + t / (g.nx * g.ny);
+    const int i = voxel(g, x, y, z);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      v[0] += (double)fields[k * N + i] * tfields[k * N + i];
+      v[1] += (double)fields[(3 + k) * N + i] * tfields[(3 + k) * N + i];
+    }
+  }
+  block_sum_store<2>(v, partials + 2 * blockIdx.x);
+}
+
+}  // namespace cpic

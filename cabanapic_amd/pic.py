@@ -61,11 +61,25 @@ class Simulation:
         h = ctypes.c_void_p()
         check(self.lib.cpic_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
+        self._owned = True
+
+    @classmethod
+    def _view(cls, handle, nx, ny, nz, dtype):
+        """A Simulation over a context owned by another object."""
+        obj = cls.__new__(cls)
+        obj.lib = _capi.load()
+        obj.dtype = np.dtype(dtype)
+        obj.nx, obj.ny, obj.nz = int(nx), int(ny), int(nz)
+        obj.ncells = (obj.nx + 2) * (obj.ny + 2) * (obj.nz + 2)
+        obj._h = handle
+        obj._owned = False
+        return obj
 
     # -- lifetime -------------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            self.lib.cpic_destroy(self._h)
+            if getattr(self, "_owned", True):
+                self.lib.cpic_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -215,6 +229,88 @@ class Simulation:
         return f.value
 
 
+class TangentSimulation:
+    """The step and its forward-mode derivative with respect to one deck
+    parameter (the cpic_tangent_* entry points).  `primal` and `tangent` are
+    Simulation views of the two field contexts (fields, interpolators,
+    accumulators, energies); the particles and their tangents live here."""
+
+    def __init__(self, nx, ny, nz, max_particles, dtype=np.float32, num_ghosts=1):
+        self.lib = _capi.load()
+        self.dtype = np.dtype(dtype)
+        if self.dtype not in (np.float32, np.float64):
+            raise ValueError("dtype must be float32 or float64")
+        cfg = _capi.Config(self.dtype.itemsize, num_ghosts, int(nx), int(ny), int(nz),
+                           int(max_particles))
+        h = ctypes.c_void_p()
+        check(self.lib.cpic_tangent_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        p, t = ctypes.c_void_p(), ctypes.c_void_p()
+        check(self.lib.cpic_tangent_contexts(h, ctypes.byref(p), ctypes.byref(t)))
+        self.primal = Simulation._view(p, nx, ny, nz, self.dtype)
+        self.tangent = Simulation._view(t, nx, ny, nz, np.float64)
+        self._n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.primal._h = self.tangent._h = None
+            self.lib.cpic_tangent_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload_particles(self, x, y, z, ux, uy, uz, w, cell, tangent):
+        """tangent: (6, n) derivatives of x, y, z, ux, uy, uz."""
+        arrs = [np.ascontiguousarray(a, self.dtype) for a in (x, y, z, ux, uy, uz, w)]
+        c = np.ascontiguousarray(cell, np.int32)
+        n = c.size
+        t = np.ascontiguousarray(tangent, np.float64)
+        if any(a.size != n for a in arrs) or t.shape != (6, n):
+            raise ValueError("particle arrays differ in length")
+        check(self.lib.cpic_tangent_upload_particles(self._h, n, *[_ptr(a) for a in arrs],
+                                                     _ptr(c), _ptr(t)))
+        self._n = n
+
+    def download_particles(self):
+        n = self._n
+        out = {k: np.empty(n, self.dtype) for k in PARTICLE_MEMBERS}
+        out["cell"] = np.empty(n, np.int32)
+        out["tangent"] = np.empty((6, n), np.float64)
+        check(self.lib.cpic_tangent_download_particles(
+            self._h, n, *[_ptr(out[k]) for k in PARTICLE_MEMBERS], _ptr(out["cell"]),
+            _ptr(out["tangent"])))
+        return out
+
+    def push(self, qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, boundary=BOUNDARY_PERIODIC):
+        check(self.lib.cpic_tangent_push(self._h, float(qdt_2mc), float(cdt_dx),
+                                         float(cdt_dy), float(cdt_dz), float(qsp),
+                                         int(boundary)))
+
+    def step(self, params, nsteps):
+        """Runs nsteps; returns the (nsteps, 4) history [E, B, dE, dB]."""
+        if isinstance(params, dict):
+            params = make_step_params(**params)
+        hist = np.full((int(nsteps), 4), np.nan)
+        check(self.lib.cpic_tangent_step(self._h, ctypes.byref(params), int(nsteps),
+                                         _ptr(hist)))
+        return hist
+
+    def last_push_movers(self):
+        n = ctypes.c_int64()
+        check(self.lib.cpic_tangent_last_push_movers(self._h, ctypes.byref(n)))
+        return n.value
+
+
 def make_step_params(qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, dx, dy, dz, dt, px, py, pz,
                      dt_eps0, me=1.0, c=1.0, boundary=BOUNDARY_PERIODIC, energy_interval=1,
                      compact_interval=0, kinetic_interval=0):
@@ -224,5 +320,5 @@ def make_step_params(qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, dx, dy, dz, dt, px, p
                       int(energy_interval), int(compact_interval), int(kinetic_interval))
 
 
-__all__ = ["Simulation", "make_step_params", "CpicError", "PARTICLE_MEMBERS", "FIELD_MEMBERS",
+__all__ = ["Simulation", "TangentSimulation", "make_step_params", "CpicError", "PARTICLE_MEMBERS", "FIELD_MEMBERS",
            "INTERP_MEMBERS"]

@@ -186,6 +186,55 @@ int cpic_stage_times(cpic_context *ctx, int64_t *counts, double *ms);
 int cpic_last_push_movers(cpic_context *ctx, int64_t *movers);
 int cpic_device_errors(cpic_context *ctx, int32_t *flags);
 
+/* ---- tangent-linear step (forward-mode derivative) ---------------------- */
+
+/* The reference has no derivative of its step; these entry points add one.
+ * A tangent context propagates, next to the primal state, its derivative
+ * with respect to one deck parameter theta: particles carry d(x, y, z, ux,
+ * uy, uz)/dtheta and an FP64 tangent field context carries dE, dB, dJ
+ * /dtheta.  theta enters through the initial tangents the caller uploads
+ * (for example d(ux)/d(drift) of every particle, or a uniform dB_z = 1 in
+ * the tangent fields for the guide field).  The primal step is the one of
+ * cpic_step with the push of src/push.h:65-295 and src/move_p.h:93-371 run
+ * one thread per particle with atomic deposits: positions, momenta and cells
+ * are bit-identical to cpic_push's, accumulators and fields agree up to the
+ * order of the floating-point sums.  The derivative is that of the piece of
+ * the step each particle is on (face crossings held as taken).
+ *
+ * cpic_tangent_contexts returns the primal context (fields, interpolators,
+ * accumulators in the configured real_t; no particles) and the tangent
+ * context (real_size 8): upload/download fields and call cpic_energies on
+ * them with the entry points above.  Both use the primal context's stream. */
+typedef struct cpic_tangent cpic_tangent;
+int cpic_tangent_create(const cpic_config *cfg, cpic_tangent **out);
+int cpic_tangent_destroy(cpic_tangent *tg);
+int cpic_tangent_contexts(cpic_tangent *tg, cpic_context **primal,
+                          cpic_context **tangent);
+/* Particles as for cpic_upload_particles (any order, interior cells), and
+ * their tangents: 6 x n doubles, tangents of x, y, z, ux, uy, uz in turn. */
+int cpic_tangent_upload_particles(cpic_tangent *tg, int64_t n, const void *x,
+                                  const void *y, const void *z,
+                                  const void *ux, const void *uy,
+                                  const void *uz, const void *w,
+                                  const int32_t *cell, const double *tangent);
+/* Particles in upload order; NULL arrays are skipped. */
+int cpic_tangent_download_particles(cpic_tangent *tg, int64_t n, void *x,
+                                    void *y, void *z, void *ux, void *uy,
+                                    void *uz, void *w, int32_t *cell,
+                                    double *tangent);
+/* One push with its tangent (adds to both accumulator sets). */
+int cpic_tangent_push(cpic_tangent *tg, double qdt_2mc, double cdt_dx,
+                      double cdt_dy, double cdt_dz, double qsp,
+                      int32_t boundary);
+/* nsteps steps of example/example.cpp:216-279 with their tangents.  hist
+ * (may be NULL) receives 4 doubles per step: E and B energies
+ * (src/fields.h:556-615) and their derivatives sum E.dE, sum B.dB, in
+ * FP64.  The energy_interval, compact_interval and kinetic_interval fields
+ * of sp are not used.  Synchronous. */
+int cpic_tangent_step(cpic_tangent *tg, const cpic_step_params *sp,
+                      int64_t nsteps, double *hist);
+int cpic_tangent_last_push_movers(cpic_tangent *tg, int64_t *movers);
+
 /* ---- deck support (host only, no device call) -------------------------- */
 
 /* Synthetic plasma loader for the decks of the benchmark and test

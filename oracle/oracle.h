@@ -7,7 +7,10 @@
  *
  * Every function is compiled twice from oracle_impl.inc: suffix _f for
  * real_t = float and _d for real_t = double, mirroring the reference's
- * compile-time REAL_TYPE switch (src/types.h:4-8).
+ * compile-time REAL_TYPE switch (src/types.h:4-8).  A third instantiation,
+ * suffix _ds (oracle_smooth.c), is the double build with a double sqrt in
+ * place of the reference's sqrtf: a smooth function of its inputs, used only
+ * as the finite-difference reference of the tangent-linear step.
  *
  * Storage conventions (shared with the product's C-ABI so tests can pass the
  * same host buffers to both):
@@ -74,6 +77,7 @@ extern "C" {
 
 ORA_DECLARE(float, _f)
 ORA_DECLARE(double, _d)
+ORA_DECLARE(double, _ds)
 
 #undef ORA_DECLARE
 

@@ -38,7 +38,7 @@ def _declare(L):
     P = ctypes.c_void_p
     Lg = ctypes.c_long
     I = ctypes.c_int
-    for sfx, R in (("_f", ctypes.c_float), ("_d", ctypes.c_double)):
+    for sfx, R in (("_f", ctypes.c_float), ("_d", ctypes.c_double), ("_ds", ctypes.c_double)):
         g = lambda n: getattr(L, n + sfx)
         g("ora_load_interpolator").argtypes = [P, P, Lg, Lg, Lg]
         g("ora_clear_accumulator").argtypes = [P, Lg]
@@ -98,6 +98,11 @@ class OracleState:
         return [_p(self.p[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")] + [_p(self.cell)]
 
     def fn(self, name):
+        # smooth: the FP64 oracle with a double sqrt in place of the
+        # reference's sqrtf (oracle/oracle_smooth.c), for finite differences
+        if getattr(self, "smooth", False):
+            assert self.dtype == np.float64
+            return getattr(lib(), name + "_ds")
         return getattr(lib(), name + sfx(self.dtype))
 
     def load_interpolator(self):
