@@ -74,6 +74,32 @@ def run_gpu(deck, steps, warmup, compact_interval, history):
     return t1 - t0, hist, stages, errs
 
 
+def run_tangent(deck, steps, warmup):
+    """The tangent-linear step (derivative with respect to a drift of ux)
+    on the same workload; returns seconds for `steps` steps."""
+    from cabanapic_amd import TangentSimulation
+    parts = deck.load_particles()
+    n = deck.num_particles
+    tan = np.zeros((6, n))
+    tan[3] = 1.0
+    ts = TangentSimulation(deck.nx, deck.ny, deck.nz, n, deck.R)
+    ts.upload_particles(*[parts[k] for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
+                        parts["cell"], tan)
+    del parts, tan
+    sp = deck.step_params(energy_interval=1)
+    if warmup:
+        ts.step(sp, warmup)
+    ts.primal.synchronize()
+    t0 = time.perf_counter()
+    hist = ts.step(sp, steps)
+    t1 = time.perf_counter()
+    errs = ts.primal.device_errors()
+    ts.close()
+    if errs or not np.isfinite(hist).all():
+        raise SystemExit(f"tangent step failed (device error flags {errs})")
+    return t1 - t0
+
+
 def cpu_baseline(deck, nsteps, nthreads):
     """The oracle's OpenMP step on the full workload for nsteps steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -108,6 +134,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=30,
                     help="steps of the CPU baseline (0 = skip)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--tangent-steps", type=int, default=5,
+                    help="timed steps of the tangent-linear step (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (torch), the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -171,6 +199,14 @@ def main():
         extra.update(fp64_value=v64, fp32_over_fp64=round(value / v64, 3),
                      field_energy_rel_err_vs_fp64=float(rel.max()),
                      field_energy_window=f"steps {args.warmup + 1}-{args.warmup + nwin}")
+
+    if args.tangent_steps > 0:
+        tt = run_tangent(d32, args.tangent_steps, 1)
+        tms = 1e3 * tt / args.tangent_steps
+        print(f"# tangent fp32: {tms:.3f} ms/step ({tms / ms_step:.2f} x the step)",
+              file=sys.stderr, flush=True)
+        extra.update(tangent_ms_per_step=round(tms, 3),
+                     tangent_over_forward=round(tms / ms_step, 3))
 
     cpu = None
     if args.cpu_steps > 0:

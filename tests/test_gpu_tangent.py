@@ -62,13 +62,24 @@ def assert_same_bits(a, b, what):
     assert n == 0, f"{what}: {n} of {a.size} differ"
 
 
+def sort_by_cell(st):
+    order = np.argsort(st.cell, kind="stable")
+    st.cell = np.ascontiguousarray(st.cell[order])
+    st.p = {m: np.ascontiguousarray(v[order]) for m, v in st.p.items()}
+
+
 @pytest.mark.parametrize("dtype", DTYPES, ids=_name)
-def test_tangent_push_primal_bitwise(dtype):
+@pytest.mark.parametrize("order", ["shuffled", "sorted"])
+def test_tangent_push_primal_bitwise(dtype, order):
     """One push of the tangent path with zero tangents: particles equal the
     oracle's push bit for bit, in upload order, and the movers count with
     them; accumulators agree to the reordered-sum bound; every tangent
-    stays exactly zero."""
-    st, k = states.random_state(6, 5, 4, 8, dtype, seed=131, umax=0.9)
+    stays exactly zero.  Particles in random cell order, and sorted by cell
+    (the deposits of a cell are then summed over runs of up to 24 lanes of a
+    wave before the atomics)."""
+    st, k = states.random_state(6, 5, 4, 24, dtype, seed=131, umax=0.9)
+    if order == "sorted":
+        sort_by_cell(st)
     with tangent_from_state(st) as ts:
         for sim in (ts.primal, ts.tangent):
             sim.load_interpolator_array()
@@ -98,6 +109,7 @@ def test_tangent_push_matches_central_differences():
     central differences of the smooth FP64 oracle's push (eps = 1e-6), to
     1e-6 relative."""
     st, k = states.random_state(5, 4, 4, 10, np.float64, seed=141, umax=0.9)
+    sort_by_cell(st)
     rng = np.random.default_rng(142)
     tfields = rng.uniform(-1, 1, st.fields.shape)
     tpart = rng.uniform(-1, 1, (6, st.np))
