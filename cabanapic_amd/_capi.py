@@ -72,6 +72,7 @@ SIGNATURES = {
     "cpic_load_interpolator_array": (ctypes.c_int, [_P]),
     "cpic_clear_accumulator_array": (ctypes.c_int, [_P]),
     "cpic_push": (ctypes.c_int, [_P, _D, _D, _D, _D, _D, _I32]),
+    "cpic_uncenter_particles": (ctypes.c_int, [_P, _D]),
     "cpic_unload_accumulator_array": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "cpic_advance_b": (ctypes.c_int, [_P, _D, _D, _D]),
     "cpic_advance_e": (ctypes.c_int, [_P, _D, _D, _D, _D]),

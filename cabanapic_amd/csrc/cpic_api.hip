@@ -808,6 +808,19 @@ int cpic_push(cpic_context* c, double qdt_2mc, double cdt_dx, double cdt_dy, dou
   });
 }
 
+int cpic_uncenter_particles(cpic_context* c, double qdt_2mc) {
+  if (int s = check_ctx(c)) return s;
+  if (c->np == 0 || c->D == 0) return CPIC_OK;
+  return with_real(c, [&](auto r) {
+    using R = decltype(r);
+    k_uncenter_columns<R><<<cdiv(c->nc, kBlock), kBlock, 0, c->stream>>>(
+        (R*)c->cols[c->cur], c->cnt[c->cur], c->mask[c->cur], (const R*)c->interp, c->cg, c->D,
+        (int)c->nc_pad, (R)qdt_2mc);
+    CHECK_LAUNCH();
+    return CPIC_OK;
+  });
+}
+
 int cpic_unload_accumulator_array(cpic_context* c, double dx, double dy, double dz, double dt) {
   if (int s = check_ctx(c)) return s;
   return with_real(c, [&](auto r) {

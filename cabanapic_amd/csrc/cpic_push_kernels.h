@@ -1128,6 +1128,61 @@ __global__ void __launch_bounds__(kBlock) k_spill_pass(SpillPass<R> a) {
   }
 }
 
+// uncenter_particles, src/uncenter_p.h:4-105: for every particle, a
+// backward half rotation in B with qdt_4mc = -0.5 qdt_2mc and gamma of the
+// momentum before it (a real_t sqrt, not the push's narrowed sqrtf), then
+// u += hax with the forward qdt_2mc (uncenter_p.h:91-93 adds the forward
+// half kick, where VPIC subtracts it).  One lane per cell, in place on the
+// current columns; positions and cells are unchanged.
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_uncenter_columns(R* __restrict__ cols, const int* __restrict__ cnt,
+                   const unsigned long long* __restrict__ mask, const R* __restrict__ interp,
+                   CellGrid g, int D, int nc_pad, R qdt_2mc) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= g.nc) return;
+  typedef typename Vec4T<R>::type V4;
+  const V4* f = reinterpret_cast<const V4*>(interp + (size_t)interior_to_voxel(g, ci) * kInterpStride);
+  const V4 e0 = f[0], e1 = f[1], e2 = f[2], b0 = f[3], b1 = f[4];
+  const R qdt_4mc = -0.5 * qdt_2mc;
+  const R one = 1.;
+  const R one_third = 1. / 3.;
+  const R two_fifteenths = 2. / 15.;
+  const int n = cnt[ci];
+  unsigned long long m = 0;
+  for (int j = 0; j < n; ++j) {
+    if ((j & 63) == 0) m = mask[(size_t)(j >> 6) * nc_pad + ci];
+    if (!((m >> (j & 63)) & 1ull)) continue;
+    V4* slot = reinterpret_cast<V4*>(cols + col_slot(ci, j, D) * kSlotLen);
+    V4 pa = slot[0], pb = slot[1];
+    const R dx = pa.x, dy = pa.y, dz = pa.z;
+    const R hax = qdt_2mc * ((e0.x + dy * e0.y) + dz * (e0.z + dy * e0.w));
+    const R hay = qdt_2mc * ((e1.x + dz * e1.y) + dx * (e1.z + dz * e1.w));
+    const R haz = qdt_2mc * ((e2.x + dx * e2.y) + dy * (e2.z + dx * e2.w));
+    const R cbx = b0.x + dx * b0.y;
+    const R cby = b0.z + dy * b0.w;
+    const R cbz = b1.x + dz * b1.y;
+    R ux = pa.w, uy = pb.x, uz = pb.y;
+    R v0 = qdt_4mc / sqrt(one + (ux * ux + (uy * uy + uz * uz)));
+    R v1 = cbx * cbx + (cby * cby + cbz * cbz);
+    R v2 = (v0 * v0) * v1;
+    const R v3 = v0 * (one + v2 * (one_third + v2 * two_fifteenths));
+    R v4 = v3 / (one + v1 * (v3 * v3));
+    v4 += v4;
+    v0 = ux + v3 * (uy * cbz - uz * cby);
+    v1 = uy + v3 * (uz * cbx - ux * cbz);
+    v2 = uz + v3 * (ux * cby - uy * cbx);
+    ux += v4 * (v1 * cbz - v2 * cby);
+    uy += v4 * (v2 * cbx - v0 * cbz);
+    uz += v4 * (v0 * cby - v1 * cbx);
+    pa.w = ux + hax;
+    pb.x = uy + hay;
+    pb.y = uz + haz;
+    slot[0] = pa;
+    slot[1] = pb;
+  }
+}
+
 // sum w (gamma - 1) over the particles of the columns -> partials[block]
 template <class R>
 __global__ void __launch_bounds__(kBlock)

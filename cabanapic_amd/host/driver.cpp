@@ -143,10 +143,6 @@ int main(int argc, char** argv) {
     return 0;
   }
 
-  if (deck.perform_uncenter) {
-    std::fprintf(stderr, "cbnpic: perform_uncenter is not supported by this build\n");
-    return 1;
-  }
   if (boundary != Boundary::Periodic) {
     // the reference's field solver exits for anything but periodic
     // boundaries (src/fields.h:21-25,113-117)
@@ -169,6 +165,12 @@ int main(int argc, char** argv) {
                                   particles.data(VelocityZ), particles.data(Weight),
                                   particles.cells()));
   CPIC_CALL(cpic_upload_fields(ctx, fields.data()));
+
+  // example/example.cpp:204-213
+  if (deck.perform_uncenter) {
+    CPIC_CALL(cpic_load_interpolator_array(ctx));
+    CPIC_CALL(cpic_uncenter_particles(ctx, qdt_2mc));
+  }
 
   if (stages) {
     for (int step = 1; step <= num_steps; step++) {

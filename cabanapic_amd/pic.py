@@ -166,6 +166,9 @@ class Simulation:
         check(self.lib.cpic_push(self._h, float(qdt_2mc), float(cdt_dx), float(cdt_dy),
                                  float(cdt_dz), float(qsp), int(boundary)))
 
+    def uncenter_particles(self, qdt_2mc):
+        check(self.lib.cpic_uncenter_particles(self._h, float(qdt_2mc)))
+
     def unload_accumulator_array(self, dx, dy, dz, dt):
         check(self.lib.cpic_unload_accumulator_array(self._h, float(dx), float(dy), float(dz),
                                                      float(dt)))

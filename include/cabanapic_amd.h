@@ -119,6 +119,14 @@ int cpic_clear_accumulator_array(cpic_context *ctx);
 int cpic_push(cpic_context *ctx, double qdt_2mc, double cdt_dx,
               double cdt_dy, double cdt_dz, double qsp, int32_t boundary);
 
+/* uncenter_particles, src/uncenter_p.h:4-105, run by example/example.cpp:
+ * 204-213 before the step loop when the deck sets perform_uncenter: every
+ * particle's momentum takes a backward half rotation in B (qdt_4mc =
+ * -0.5 qdt_2mc) and then the forward half E kick u += hax, as the
+ * reference writes it (uncenter_p.h:91-93).  Uses the interpolators as
+ * loaded; positions and cells are unchanged. */
+int cpic_uncenter_particles(cpic_context *ctx, double qdt_2mc);
+
 /* unload_accumulator_array, src/accumulator.h:23-34,
  * src/accumulator.cpp:45-122 */
 int cpic_unload_accumulator_array(cpic_context *ctx, double dx, double dy,

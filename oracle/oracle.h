@@ -55,6 +55,9 @@ extern "C" {
                        R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp,         \
                        long nx, long ny, long nz, int boundary,                \
                        int nthreads);                                          \
+  void ora_uncenter##S(const R *x, const R *y, const R *z, R *ux, R *uy,       \
+                       R *uz, const int32_t *cell, long np, const R *interp,   \
+                       R qdt_2mc, long nx, long ny, long nz);                  \
   void ora_unload_accumulator##S(R *fields, const R *acc, long nx, long ny,    \
                                  long nz, R dx, R dy, R dz, R dt);             \
   void ora_fold_ghosts##S(R *sx, R *sy, R *sz, long nx, long ny, long nz);     \

@@ -49,6 +49,7 @@ def _declare(L):
         g("ora_push_omp").argtypes = [P] * 8 + [Lg, P, P] + [R] * 5 + [Lg, Lg, Lg, I, I]
         g("ora_push_omp").restype = Lg
         g("ora_unload_accumulator").argtypes = [P, P, Lg, Lg, Lg, R, R, R, R]
+        g("ora_uncenter").argtypes = [P] * 7 + [Lg, P, R, Lg, Lg, Lg]
         g("ora_fold_ghosts").argtypes = [P, P, P, Lg, Lg, Lg]
         g("ora_copy_ghosts").argtypes = [P, P, P, Lg, Lg, Lg]
         g("ora_advance_b").argtypes = [P, R, R, R, Lg, Lg, Lg]
@@ -124,6 +125,10 @@ class OracleState:
         return self.fn("ora_push")(*self._pp(), self.np, _p(self.interp), _p(self.acc),
                                    qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp,
                                    self.nx, self.ny, self.nz, boundary)
+
+    def uncenter(self, qdt_2mc):
+        self.fn("ora_uncenter")(*self._pp()[:6], _p(self.cell), self.np, _p(self.interp),
+                                qdt_2mc, self.nx, self.ny, self.nz)
 
     def unload(self, dx, dy, dz, dt):
         self.fn("ora_unload_accumulator")(_p(self.fields), _p(self.acc), self.nx, self.ny, self.nz,

@@ -183,6 +183,26 @@ def test_energies(dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_uncenter_bitwise(dtype, grid):
+    """uncenter_particles (src/uncenter_p.h:4-105) on loaded interpolators:
+    momenta bit-identical to the oracle's, positions and cells unchanged."""
+    nx, ny, nz, ppc = grid
+    st, k = states.random_state(nx, ny, nz, ppc, dtype, seed=61, umax=0.9)
+    with sim_from_state(st) as sim:
+        sim.load_interpolator_array()
+        sim.uncenter_particles(k["qdt_2mc"])
+        got = sim.download_particles()
+    before = states.oracle_particles(st.copy())
+    st.load_interpolator()
+    st.uncenter(k["qdt_2mc"])
+    assert_particles_bitwise(got, st)
+    for m in ("x", "y", "z"):
+        assert_bitwise(states.canonical(got)[m], states.canonical(before)[m], m)
+    assert not np.array_equal(states.canonical(got)["ux"], states.canonical(before)["ux"])
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
 def test_one_step_particles_bitwise(dtype):
     """A full fused step: the push sees identical interpolators, so the
     particles match bit for bit; fields agree to the reordered-sum bound."""

@@ -9,6 +9,10 @@ exercised there).  These checks close that gap:
   and leaves the other components untouched; uniform B conserves |u|, keeps
   u parallel to B and rotates u about B by -2 qdt_2mc |B| / gamma up to the
   x^7 truncation of the tan series; the rotation sense is that of q u x B.
+* uncenter_particles (src/uncenter_p.h:4-105): uniform E with B = 0 adds
+  exactly the forward half kick qdt_2mc E (the reference's sign, which VPIC
+  reverses); uniform B conserves |u| and u.B and rotates u back by half the
+  push's angle, +qdt_2mc |B| / gamma within the tan-series truncation.
 * move_p (src/move_p.h:59-227): free streaming advances the unwrapped
   coordinate 2(i-1) + x + 1 by 2 u cdt / gamma per step on every axis,
   through any number of face, edge and corner crossings and periodic wraps;
@@ -184,6 +188,69 @@ def test_uniform_b_rotation(x):
     want = -2 * qdt_2mc * bmag / gamma
     # gamma comes from a float sqrt in the double build: 1.2e-7 relative
     bound = 2 * (17 / 315) * x ** 7 + 1.2e-7 * np.abs(want)
+    assert np.all(np.abs(ang - want) <= bound), np.abs(ang - want).max()
+
+
+# ----------------------------------------------------------------------------
+# uncenter_particles
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_uncenter_uniform_e_adds_forward_half_kick(dtype):
+    """With B = 0 the rotation is the identity, so u' = u + qdt_2mc E in
+    real_t arithmetic, bit for bit: the sign is the reference's +hax
+    (uncenter_p.h:91-93), not the backward kick VPIC applies."""
+    rng = np.random.default_rng(11)
+    n = 48
+    st = _single(np_=n, dtype=dtype)
+    E = rng.uniform(-1, 1, 3)
+    _uniform_fields(st, e=E)
+    st.cell[:] = _voxel(st, 3, 1, 2)
+    for m in ("x", "y", "z"):
+        st.p[m][:] = rng.uniform(-1, 1, n)
+    for m in ("ux", "uy", "uz"):
+        st.p[m][:] = rng.uniform(-0.5, 0.5, n)
+    u0 = {m: st.p[m].copy() for m in ("ux", "uy", "uz")}
+    qdt_2mc = dtype(-0.03)
+    st.load_interpolator()
+    st.uncenter(qdt_2mc)
+    for c, m in enumerate(("ux", "uy", "uz")):
+        want = u0[m] + qdt_2mc * dtype(E[c])
+        assert want.dtype == dtype and np.array_equal(st.p[m], want), m
+    for m in ("x", "y", "z"):
+        assert not np.array_equal(st.p[m], np.zeros(n, dtype))
+
+
+@pytest.mark.parametrize("x", [0.02, 0.2])
+def test_uncenter_uniform_b_rotates_back_half_angle(x):
+    """With E = 0: |u| and u.B conserved; the angle about B is
+    +qdt_2mc |B| / gamma (minus half the push's angle, qdt_4mc = -0.5 qdt_2mc)
+    within the tan-series truncation 2 (17/315) x^7 with x = |qdt_4mc B| /
+    gamma.  A sign error in qdt_4mc flips the angle; a full qdt_2mc doubles
+    it."""
+    rng = np.random.default_rng(13)
+    n = 24
+    bdir = rng.normal(size=3)
+    bdir /= np.linalg.norm(bdir)
+    u = rng.normal(size=(n, 3)) * 0.4
+    gamma = np.sqrt(1 + (u * u).sum(1))
+    qdt_2mc = -0.05
+    bmag = x * gamma / abs(0.5 * qdt_2mc)
+    got = np.zeros_like(u)
+    for p in range(n):
+        one = _single()
+        _uniform_fields(one, b=bdir * bmag[p])
+        one.cell[:] = _voxel(one, 1, 2, 3)
+        one.p["ux"][:], one.p["uy"][:], one.p["uz"][:] = u[p]
+        one.load_interpolator()
+        one.uncenter(qdt_2mc)
+        got[p] = one.p["ux"][0], one.p["uy"][0], one.p["uz"][0]
+    assert np.allclose(np.linalg.norm(got, axis=1), np.linalg.norm(u, axis=1), rtol=1e-14, atol=0)
+    assert np.allclose(got @ bdir, u @ bdir, rtol=0, atol=1e-14)
+    up = u - np.outer(u @ bdir, bdir)
+    gp = got - np.outer(got @ bdir, bdir)
+    ang = np.arctan2(np.cross(up, gp) @ bdir, (up * gp).sum(1))
+    want = qdt_2mc * bmag / gamma
+    bound = 2 * (17 / 315) * x ** 7 + 1e-13
     assert np.all(np.abs(ang - want) <= bound), np.abs(ang - want).max()
 
 
