@@ -82,6 +82,7 @@ SIGNATURES = {
     "cpic_sort_particles": (ctypes.c_int, [_P]),
     "cpic_step": (ctypes.c_int, [_P, ctypes.POINTER(StepParams), _I64, _P]),
     "cpic_set_timing": (ctypes.c_int, [_P, _I32]),
+    "cpic_set_timing_stages": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "cpic_stage_times": (ctypes.c_int, [_P, _P, _P]),
     "cpic_last_push_movers": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "cpic_device_errors": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),

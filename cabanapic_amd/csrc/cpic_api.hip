@@ -104,8 +104,8 @@ struct cpic_context {
   hipStream_t stream = nullptr;
   int device = 0;
 
-  // stage timing
-  bool timing = false;
+  // stage timing: bit s set = stage s bracketed by events
+  unsigned timing = 0;
   struct Pending {
     int stage;
     hipEvent_t a, b;
@@ -138,25 +138,29 @@ int get_event(cpic_context* c, hipEvent_t* e) {
     c->free_events.pop_back();
     return CPIC_OK;
   }
-  HIP_TRY(hipEventCreate(e));
+  // timing-only events: no system-scope fence when they complete, whose
+  // cache write-back and invalidation would stall the next kernel by about
+  // 10 us (the times are read only after a stream synchronisation)
+  HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableSystemFence));
   return CPIC_OK;
 }
 
-// Brackets the launches of one stage with events when timing is on.
+// Brackets the launches of one stage with events when its timing is on.
 struct StageTimer {
   cpic_context* c;
   int stage;
+  bool on;
   hipEvent_t a = nullptr, b = nullptr;
   int status = CPIC_OK;
-  StageTimer(cpic_context* ctx, int s) : c(ctx), stage(s) {
-    if (!c->timing) return;
+  StageTimer(cpic_context* ctx, int s) : c(ctx), stage(s), on((ctx->timing >> s) & 1u) {
+    if (!on) return;
     if ((status = get_event(c, &a)) != CPIC_OK) return;
     if ((status = get_event(c, &b)) != CPIC_OK) return;
     if (hipEventRecord(a, c->stream) != hipSuccess)
       status = fail(CPIC_ERR_HIP, "hipEventRecord failed");
   }
   int finish() {
-    if (!c->timing || status != CPIC_OK) return status;
+    if (!on || status != CPIC_OK) return status;
     HIP_TRY(hipEventRecord(b, c->stream));
     c->pending.push_back({stage, a, b});
     return CPIC_OK;
@@ -952,10 +956,14 @@ int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, doubl
 }
 
 int cpic_set_timing(cpic_context* c, int32_t enabled) {
+  return cpic_set_timing_stages(c, enabled ? (1u << CPIC_NUM_STAGES) - 1 : 0u);
+}
+
+int cpic_set_timing_stages(cpic_context* c, uint32_t stages) {
   if (int s = check_ctx(c)) return s;
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (int s = collect_timing(c)) return s;
-  c->timing = enabled != 0;
+  c->timing = stages & ((1u << CPIC_NUM_STAGES) - 1);
   for (int k = 0; k < CPIC_NUM_STAGES; ++k) {
     c->stage_count[k] = 0;
     c->stage_ms[k] = 0;

@@ -100,15 +100,42 @@ __device__ __forceinline__ void deposit_dual(const TangentPush<R>& a, int v, con
   }
 }
 
-// One thread per particle: the push of src/push.h:65-295 with the move_p
-// loop of src/move_p.h:93-371, on dual numbers, deposits added atomically
-// (primal to acc, tangent to tacc).  The primal positions, momenta and cells
-// equal those of the column push; the primal accumulators equal them up to
-// the order of the floating-point sums.
+// Adds o (the deposits of the calling lane into voxel key, key < 0 for
+// none) to the accumulators, summed first over each run of consecutive lanes
+// of the wave with the same key (a segmented inclusive scan; the last lane
+// of a run adds the run's total).  Particles of one cell that sit in
+// consecutive lanes thus cost one atomic per value instead of one per lane.
+// Every lane of the wave must call it.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= a.n) return;
+__device__ __forceinline__ void deposit_runs(const TangentPush<R>& a, int key, Dual<R> o[12]) {
+  const int lane = threadIdx.x & 63;
+  const int before = __shfl_up(key, 1, 64);
+  int head = (lane == 0 || before != key) ? 1 : 0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int h = __shfl_up(head, off, 64);
+    const bool add = lane >= off && !head;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      const R yv = __shfl_up(o[k].v, off, 64);
+      const double yt = __shfl_up(o[k].t, off, 64);
+      if (add) {
+        o[k].v = yv + o[k].v;
+        o[k].t = yt + o[k].t;
+      }
+    }
+    if (lane >= off) head |= h;
+  }
+  const int after = __shfl_down(key, 1, 64);
+  if (key >= 0 && (lane == 63 || after != key)) deposit_dual(a, key, o);
+}
+
+// The push of particle p (nothing for p >= n); returns the voxel of its own
+// cell, whose deposits are left in own[], or -1.
+template <class R>
+__device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a, int p,
+                                                     Dual<R> own[12]) {
+  if (p >= a.n) return -1;
   typedef Dual<R> D;
   typedef typename Vec4T<R>::type V4;
   typedef double T4 __attribute__((ext_vector_type(4)));
@@ -119,7 +146,7 @@ __global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
   if (v0 < 0 || v0 >= a.ncells || ix < 0 || ix >= a.g.nx || iy < 0 || iy >= a.g.ny || iz < 0 ||
       iz >= a.g.nz) {
     atomicOr(a.err, kErrBadCell);
-    return;
+    return -1;
   }
   int ci = ix + a.g.nx * (iy + a.g.ny * iz);
 
@@ -154,9 +181,7 @@ __global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
   D pos[3];
   if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
     const D v5 = q * ddx * ddy * ddz * one_third;
-    D o[12];
-    streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
-    deposit_dual(a, v0, o);
+    streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, own);
     pos[0] = nx;
     pos[1] = ny;
     pos[2] = nz;
@@ -166,18 +191,14 @@ __global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
     pos[1] = dy;
     pos[2] = dz;
     D rem[3] = {ddx, ddy, ddz};
-    bool done = false;
-    for (int it = 0; it < kMaxStreaks; ++it) {
-      D o[12];
-      const int d = streak(pos, rem, q, o);
-      deposit_dual(a, interior_to_voxel(a.g, ci), o);
-      if (d < 0) {
-        done = true;
-        break;
-      }
+    int d = streak(pos, rem, q, own);  // the first streak, in the own cell
+    for (int it = 1; it < kMaxStreaks && d >= 0; ++it) {
       ci = face_neighbor(a.g, ci, d);
+      D o[12];
+      d = streak(pos, rem, q, o);
+      deposit_dual(a, interior_to_voxel(a.g, ci), o);
     }
-    if (!done) atomicOr(a.err, kErrStreakLimit);
+    if (d >= 0) atomicOr(a.err, kErrStreakLimit);
   }
   a.part[p] = pos[0].v;
   a.part[S + p] = pos[1].v;
@@ -192,6 +213,26 @@ __global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
   a.tpart[4 * S + p] = uy.t;
   a.tpart[5 * S + p] = uz.t;
   a.cell[p] = interior_to_voxel(a.g, ci);
+  return v0;
+}
+
+// One thread per particle: the push of src/push.h:65-295 with the move_p
+// loop of src/move_p.h:93-371, on dual numbers, deposits added atomically
+// (primal to acc, tangent to tacc).  The deposits into the particle's own
+// cell (its whole track, or the first streak of a particle that leaves) are
+// summed over the wave's runs of lanes in one cell first; the later streaks
+// of a particle that leaves are added directly.  The primal positions,
+// momenta and cells equal those of the column push; the primal accumulators
+// equal them up to the order of the floating-point sums.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  typedef Dual<R> D;
+  D own[12];  // deposits into the particle's own cell
+  // every lane takes part in the wave's run sums, so a lane without a
+  // particle (or with an invalid cell) keeps key -1 and no deposit
+  const int key = push_tangent_particle(a, p, own);
+  deposit_runs(a, key, own);
 }
 
 // sum over interior cells of E . tE and B . tB -> partials[block][2] (FP64):

@@ -212,8 +212,13 @@ class Simulation:
         return hist
 
     # -- diagnostics ------------------------------------------------------------
-    def set_timing(self, enabled):
-        check(self.lib.cpic_set_timing(self._h, 1 if enabled else 0))
+    def set_timing(self, enabled, stages=None):
+        """Times every stage (enabled), none, or the named stages only."""
+        if stages is None:
+            check(self.lib.cpic_set_timing(self._h, 1 if enabled else 0))
+        else:
+            mask = sum(1 << _capi.STAGES.index(s) for s in stages) if enabled else 0
+            check(self.lib.cpic_set_timing_stages(self._h, mask))
 
     def stage_times(self):
         counts = np.zeros(_capi.NUM_STAGES, np.int64)

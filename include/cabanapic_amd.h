@@ -174,9 +174,12 @@ int cpic_step(cpic_context *ctx, const cpic_step_params *sp, int64_t nsteps,
 /* ---- diagnostics --------------------------------------------------------- */
 
 /* Stage timing with HIP events on the context stream.  While enabled, each
- * stage launched by cpic_step (and by the single-stage entry points) is
- * bracketed by events; cpic_stage_times returns, per stage, the number of
- * timed launches and their summed milliseconds.  Stage ids: */
+ * timed stage launched by cpic_step (and by the single-stage entry points)
+ * is bracketed by events; cpic_stage_times returns, per stage, the number
+ * of timed launches and their summed milliseconds.  Each event pair delays
+ * the stream by a few microseconds.  cpic_set_timing times every stage (or
+ * none); cpic_set_timing_stages times the stages whose bit (1 << id) is set.
+ * Either call resets the totals.  Stage ids: */
 #define CPIC_STAGE_INTERP 0
 #define CPIC_STAGE_CLEAR 1
 #define CPIC_STAGE_PUSH 2
@@ -187,6 +190,7 @@ int cpic_step(cpic_context *ctx, const cpic_step_params *sp, int64_t nsteps,
 #define CPIC_STAGE_SORT 7
 #define CPIC_NUM_STAGES 8
 int cpic_set_timing(cpic_context *ctx, int32_t enabled);
+int cpic_set_timing_stages(cpic_context *ctx, uint32_t stages);
 int cpic_stage_times(cpic_context *ctx, int64_t *counts, double *ms);
 
 /* Particles that took the move_p branch in the last push, and the device

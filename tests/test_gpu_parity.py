@@ -377,3 +377,25 @@ def test_reflect_boundary_unsupported():
     with sim_from_state(st) as sim:
         with pytest.raises(CpicError):
             sim.push(**k, boundary=0)
+
+
+def test_stage_timing_masks():
+    """Stage timing counts only the stages asked for: the push alone, then
+    every stage of the fused step once per step."""
+    st, _ = states.random_state(4, 3, 3, 6, np.float32, seed=201)
+    k = states.step_consts(np.float32)
+    with sim_from_state(st) as sim:
+        sim.set_timing(True, stages=("push",))
+        sim.step(k, 3, history=False)
+        t = sim.stage_times()
+        assert t["push"][0] == 3 and t["push"][1] > 0
+        assert all(v[0] == 0 for name, v in t.items() if name != "push")
+        sim.set_timing(True)
+        sim.step(dict(k, energy_interval=1), 2, history=True)
+        t = sim.stage_times()
+        for name in ("interp", "push", "unload", "advance_e", "energy"):
+            assert t[name][0] == 2, name
+        assert t["advance_b"][0] == 4
+        sim.set_timing(False)
+        sim.step(k, 1, history=False)
+        assert all(v[0] == 0 for v in sim.stage_times().values())
