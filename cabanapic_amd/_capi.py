@@ -21,6 +21,9 @@ CPIC_ERR_DEVICE = 4
 BOUNDARY_REFLECT = 0
 BOUNDARY_PERIODIC = 1
 
+FIELD_SOLVER_EM = 0
+FIELD_SOLVER_ES_1D = 1
+
 STAGES = ("interp", "clear", "push", "unload", "advance_b", "advance_e", "energy", "sort")
 NUM_STAGES = len(STAGES)
 
@@ -77,6 +80,7 @@ SIGNATURES = {
     "cpic_advance_b": (ctypes.c_int, [_P, _D, _D, _D]),
     "cpic_advance_e": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "cpic_update_ghosts_j": (ctypes.c_int, [_P]),
+    "cpic_set_field_solver": (ctypes.c_int, [_P, _I32]),
     "cpic_energies": (ctypes.c_int, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "cpic_kinetic_energy": (ctypes.c_int, [_P, _D, _D, ctypes.POINTER(_D)]),
     "cpic_sort_particles": (ctypes.c_int, [_P]),

@@ -103,6 +103,7 @@ struct cpic_context {
 
   hipStream_t stream = nullptr;
   int device = 0;
+  int solver = CPIC_FIELD_SOLVER_EM;  // field solver of advance_b/e and energies
 
   // stage timing: bit s set = stage s bracketed by events
   unsigned timing = 0;
@@ -354,6 +355,8 @@ int launch_fold_j(cpic_context* c) {
 template <class R>
 int stage_advance_b(cpic_context* c, R px, R py, R pz) {
   StageTimer t(c, CPIC_STAGE_ADVANCE_B);
+  // ES_Field_Solver_1D::advance_b does nothing (src/fields.h:470-480)
+  if (c->solver == CPIC_FIELD_SOLVER_ES_1D) return t.finish();
   const long n = c->nx * c->ny * c->nz;
   k_advance_b<R><<<cdiv(n, kBlock), kBlock, 0, c->stream>>>((R*)c->fields, c->grid, px, py, pz);
   CHECK_LAUNCH();
@@ -372,6 +375,16 @@ int stage_ghosts_j(cpic_context* c) {
 template <class R>
 int stage_advance_e(cpic_context* c, R px, R py, R pz, R dt_eps0) {
   StageTimer t(c, CPIC_STAGE_ADVANCE_E);
+  if (c->solver == CPIC_FIELD_SOLVER_ES_1D) {
+    // ES_Field_Solver_1D::advance_e, src/fields.h:511-545
+    k_fold_j_ref<R><<<cdiv(c->nx + c->ny + c->nz, kBlock), kBlock, 0, c->stream>>>((R*)c->fields,
+                                                                                 c->grid);
+    CHECK_LAUNCH();
+    k_es_advance_e<R><<<cdiv(c->ncells, kBlock), kBlock, 0, c->stream>>>(
+        (R*)c->fields, (int)c->ncells, dt_eps0);
+    CHECK_LAUNCH();
+    return t.finish();
+  }
   int s = stage_ghosts_j<R>(c);
   if (s) return s;
   const long n = (c->nx + 1) * (c->ny + 1) * (c->nz + 1);
@@ -384,10 +397,15 @@ int stage_advance_e(cpic_context* c, R px, R py, R pz, R dt_eps0) {
 // E and B energies into out[0..1] (device), FP64.
 template <class R>
 int launch_field_energy(cpic_context* c, double* out) {
-  const long n = c->nx * c->ny * c->nz;
+  const bool es = c->solver == CPIC_FIELD_SOLVER_ES_1D;
+  const long n = es ? c->ncells : c->nx * c->ny * c->nz;
   const int blocks = std::max(1, std::min(cdiv(n, kBlock), c->n_partials));
-  k_field_energy_partial<R><<<blocks, kBlock, 0, c->stream>>>((const R*)c->fields, c->grid,
-                                                              c->partials);
+  if (es)
+    k_es_energy_partial<R><<<blocks, kBlock, 0, c->stream>>>((const R*)c->fields, c->grid,
+                                                             c->partials);
+  else
+    k_field_energy_partial<R><<<blocks, kBlock, 0, c->stream>>>((const R*)c->fields, c->grid,
+                                                                c->partials);
   CHECK_LAUNCH();
   k_reduce_final<2><<<1, kBlock, 0, c->stream>>>(c->partials, blocks, 0.5, 0.5, out);
   CHECK_LAUNCH();
@@ -847,6 +865,17 @@ int cpic_advance_e(cpic_context* c, double px, double py, double pz, double dt_e
     using R = decltype(r);
     return stage_advance_e<R>(c, (R)px, (R)py, (R)pz, (R)dt_eps0);
   });
+}
+
+int cpic_set_field_solver(cpic_context* c, int32_t solver) {
+  if (int s = check_ctx(c)) return s;
+  if (solver != CPIC_FIELD_SOLVER_EM && solver != CPIC_FIELD_SOLVER_ES_1D)
+    return fail(CPIC_ERR_INVALID, "unknown field solver %d", solver);
+  // example/example.cpp:69-74
+  if (solver == CPIC_FIELD_SOLVER_ES_1D && (c->ny > 1 || c->nz > 1))
+    return fail(CPIC_ERR_UNSUPPORTED, "the ES field solver supports 1D (ny = nz = 1) only");
+  c->solver = solver;
+  return CPIC_OK;
 }
 
 int cpic_update_ghosts_j(cpic_context* c) {

@@ -328,6 +328,71 @@ __device__ __forceinline__ void block_sum_store(double (&v)[NQ], double* out) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// The electrostatic 1D solver (ES_Field_Solver_1D, src/fields.h:467-545).
+
+// serial_update_ghosts (src/fields.h:102-271) as the reference runs it,
+// including the ghost cells its passes leave behind, which the ES solver
+// keeps (no periodic copy follows): one thread per x plane for jfx (its y
+// fold then its z fold), per y plane for jfy (z then x), per z plane for jfz
+// (x then y).  A thread touches only its own plane of its component.
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_fold_j_ref(R* __restrict__ fields, Grid g) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t N = g.ncells;
+  if (t < g.nx) {
+    R* s = fields + 6 * N;
+    const int x = t + 1;
+    for (int z = 1; z <= g.nz + 1; ++z) s[voxel(g, x, 1, z)] += s[voxel(g, x, g.ny + 1, z)];
+    for (int y = 1; y <= g.ny + 1; ++y) s[voxel(g, x, y, 1)] += s[voxel(g, x, y, g.nz + 1)];
+  } else if (t < g.nx + g.ny) {
+    R* s = fields + 7 * N;
+    const int y = t - g.nx + 1;
+    for (int x = 1; x <= g.nx + 1; ++x) s[voxel(g, x, y, 1)] += s[voxel(g, x, y, g.nz + 1)];
+    for (int z = 1; z <= g.nz + 1; ++z) s[voxel(g, 1, y, z)] += s[voxel(g, g.nx + 1, y, z)];
+  } else if (t < g.nx + g.ny + g.nz) {
+    R* s = fields + 8 * N;
+    const int z = t - g.nx - g.ny + 1;
+    for (int y = 1; y <= g.ny + 1; ++y) s[voxel(g, 1, y, z)] += s[voxel(g, g.nx + 1, y, z)];
+    for (int x = 1; x <= g.nx + 1; ++x) s[voxel(g, x, 1, z)] += s[voxel(g, x, g.ny + 1, z)];
+  }
+}
+
+// ES advance_e, src/fields.h:537-542: E += -dt_eps0 J on every voxel,
+// ghosts included
+template <class R>
+__global__ void __launch_bounds__(kBlock) k_es_advance_e(R* __restrict__ fields, int ncells,
+                                                         R dt_eps0) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ncells) return;
+  const size_t N = ncells;
+  const R cj = dt_eps0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) fields[c * N + i] = fields[c * N + i] + (-cj * fields[(6 + c) * N + i]);
+}
+
+// ES e_energy (src/fields.h:482-509): E^2 over every voxel, ghosts
+// included; with B^2 over interior cells as for the EM solver (the ES
+// solver leaves B as it is) -> partials[block][2]
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_es_energy_partial(const R* __restrict__ fields, Grid g, double* __restrict__ partials) {
+  const int n = g.ncells;
+  const size_t N = g.ncells;
+  const int X = g.sy, Y = g.sz / g.sy;
+  double v[2] = {0, 0};
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double e0 = fields[i], e1 = fields[N + i], e2 = fields[2 * N + i];
+    v[0] += e0 * e0 + e1 * e1 + e2 * e2;
+    const int x = i % X, y = (i / X) % Y, z = i / g.sz;
+    if (x >= 1 && x <= g.nx && y >= 1 && y <= g.ny && z >= 1 && z <= g.nz) {
+      const double b0 = fields[3 * N + i], b1 = fields[4 * N + i], b2 = fields[5 * N + i];
+      v[1] += b0 * b0 + b1 * b1 + b2 * b2;
+    }
+  }
+  block_sum_store<2>(v, partials + 2 * blockIdx.x);
+}
+
 // E^2 and B^2 over interior cells -> partials[block][2]
 template <class R>
 __global__ void __launch_bounds__(kBlock)

@@ -179,6 +179,11 @@ class Simulation:
     def advance_e(self, px, py, pz, dt_eps0):
         check(self.lib.cpic_advance_e(self._h, float(px), float(py), float(pz), float(dt_eps0)))
 
+    def set_field_solver(self, solver):
+        """"em" (default) or "es_1d" (ES_Field_Solver_1D)."""
+        code = {"em": _capi.FIELD_SOLVER_EM, "es_1d": _capi.FIELD_SOLVER_ES_1D}[solver]
+        check(self.lib.cpic_set_field_solver(self._h, code))
+
     def update_ghosts_j(self):
         check(self.lib.cpic_update_ghosts_j(self._h))
 

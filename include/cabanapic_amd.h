@@ -141,6 +141,20 @@ int cpic_advance_b(cpic_context *ctx, double px, double py, double pz);
 int cpic_advance_e(cpic_context *ctx, double px, double py, double pz,
                    double dt_eps0);
 
+/* Field solver used by cpic_advance_b, cpic_advance_e, cpic_energies and
+ * cpic_step: the electromagnetic Yee solver (Field_Solver<EM_Field_Solver>,
+ * the reference's default) or the electrostatic 1D solver
+ * (Field_Solver<ES_Field_Solver_1D>, src/fields.h:467-545, selected in the
+ * reference by building with ES_FIELD_SOLVER, example/example.cpp:27-33).
+ * ES: advance_b does nothing; advance_e folds the J ghosts as
+ * serial_update_ghosts does (ghost cells included) and adds -dt_eps0 J to E
+ * on every voxel; the E energy sums every voxel, ghosts included (the B
+ * energy is that of the EM solver).  ES needs ny = nz = 1
+ * (example/example.cpp:69-74), else CPIC_ERR_UNSUPPORTED.  Default EM. */
+#define CPIC_FIELD_SOLVER_EM 0
+#define CPIC_FIELD_SOLVER_ES_1D 1
+int cpic_set_field_solver(cpic_context *ctx, int32_t solver);
+
 /* serial_update_ghosts on jfx,jfy,jfz (src/fields.h:102-271) followed by
  * serial_update_ghosts_B on J (src/fields.h:11-100): the ghost update that
  * advance_e performs first, exposed on its own. */

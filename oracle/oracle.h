@@ -68,6 +68,9 @@ extern "C" {
                         long nz, R dt_eps0);                                   \
   void ora_energies##S(const R *fields, long nx, long ny, long nz,             \
                        double *e64, double *b64, R *e_real, R *b_real);        \
+  void ora_es_advance_e##S(R *fields, long nx, long ny, long nz, R dt_eps0);  \
+  void ora_es_energy##S(const R *fields, long nx, long ny, long nz,           \
+                        double *e64, R *e_real);                               \
   double ora_kinetic_energy##S(const R *ux, const R *uy, const R *uz,          \
                                const R *w, long np, R me, R c);                \
   void ora_default_particle_init##S(R *x, R *y, R *z, R *ux, R *uy, R *uz,     \

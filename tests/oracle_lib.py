@@ -55,6 +55,8 @@ def _declare(L):
         g("ora_advance_b").argtypes = [P, R, R, R, Lg, Lg, Lg]
         g("ora_advance_e").argtypes = [P, R, R, R, Lg, Lg, Lg, R]
         g("ora_energies").argtypes = [P, Lg, Lg, Lg, P, P, P, P]
+        g("ora_es_advance_e").argtypes = [P, Lg, Lg, Lg, R]
+        g("ora_es_energy").argtypes = [P, Lg, Lg, Lg, P, P]
         g("ora_kinetic_energy").argtypes = [P, P, P, P, Lg, R, R]
         g("ora_kinetic_energy").restype = ctypes.c_double
         g("ora_default_particle_init").argtypes = [P] * 8 + [Lg, Lg, Lg, R, Lg, R, R]
@@ -148,6 +150,16 @@ class OracleState:
         f = self.fields
         self.fn("ora_copy_ghosts")(_p(f[first]), _p(f[first + 1]), _p(f[first + 2]),
                                    self.nx, self.ny, self.nz)
+
+    def es_advance_e(self, dt_eps0):
+        self.fn("ora_es_advance_e")(_p(self.fields), self.nx, self.ny, self.nz, dt_eps0)
+
+    def es_energy(self):
+        e64 = ctypes.c_double()
+        er = (ctypes.c_float if self.dtype == np.float32 else ctypes.c_double)()
+        self.fn("ora_es_energy")(_p(self.fields), self.nx, self.ny, self.nz, ctypes.byref(e64),
+                                 ctypes.byref(er))
+        return e64.value, er.value
 
     def energies(self):
         e64 = ctypes.c_double()

@@ -399,3 +399,73 @@ def test_stage_timing_masks():
         sim.set_timing(False)
         sim.step(k, 1, history=False)
         assert all(v[0] == 0 for v in sim.stage_times().values())
+
+
+# (nx, ny, nz, ppc): the electrostatic solver is 1D (example/example.cpp:69-74)
+ES_GRIDS = [(16, 1, 1, 8), (5, 1, 1, 20)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+@pytest.mark.parametrize("grid", ES_GRIDS, ids=lambda g: "x".join(map(str, g)))
+def test_es_solver_stages_bitwise(dtype, grid):
+    """ES_Field_Solver_1D (src/fields.h:467-545) on random fields with
+    nonzero ghost current: advance_b leaves the fields as they are;
+    advance_e (the fold with its ghost cells, then E += -dt_eps0 J on every
+    voxel) is bit-identical to the oracle; the all-voxel E energy agrees to
+    FP64 rounding."""
+    nx, ny, nz, ppc = grid
+    st, _ = states.random_state(nx, ny, nz, ppc, dtype, seed=211)
+    k = states.step_consts(dtype)
+    with sim_from_state(st) as sim:
+        sim.set_field_solver("es_1d")
+        sim.advance_b(k["px"], k["py"], k["pz"])
+        assert_bitwise(sim.download_fields(), st.fields, "fields after ES advance_b")
+        sim.advance_e(k["px"], k["py"], k["pz"], k["dt_eps0"])
+        f = sim.download_fields()
+        e, b = sim.energies()
+    st.es_advance_e(k["dt_eps0"])
+    assert_bitwise(f, st.fields, "fields after ES advance_e")
+    e64, _ = st.es_energy()
+    b64 = st.energies()[1]
+    assert abs(e - e64) <= 1e-12 * e64 and abs(b - b64) <= 1e-12 * b64
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_es_fused_steps_follow_the_oracle(dtype):
+    """cpic_step with the ES solver against the oracle's ES stage loop
+    (load_interpolator, push, unload, ES advance_e): particles bit for bit
+    after the first step, energies within the reordered-J band over 20
+    steps."""
+    nx, ny, nz, ppc = 24, 1, 1, 16
+    st, _ = states.random_state(nx, ny, nz, ppc, dtype, seed=221, umax=0.3)
+    st.fields[:] *= dtype(0.01)
+    k = states.step_consts(dtype)
+    o = st.copy()
+    with sim_from_state(st) as sim:
+        sim.set_field_solver("es_1d")
+        h1 = sim.step(dict(k, energy_interval=1), 1)
+        got = sim.download_particles()
+        h = sim.step(dict(k, energy_interval=1), 19)
+    hist = np.vstack([h1, h])
+    ref = []
+    for step in range(20):
+        o.load_interpolator()
+        o.clear_accumulator()
+        o.push(k["qdt_2mc"], k["cdt_dx"], k["cdt_dy"], k["cdt_dz"], k["qsp"])
+        o.unload(k["dx"], k["dy"], k["dz"], k["dt"])
+        o.es_advance_e(k["dt_eps0"])
+        if step == 0:
+            assert_particles_bitwise(got, o)
+        ref.append(o.es_energy()[0])
+    ref = np.array(ref)
+    rel = np.abs(hist[:, 0] - ref) / np.abs(ref)
+    assert rel.max() < (1e-3 if dtype == np.float32 else 1e-9), rel.max()
+
+
+def test_es_solver_needs_1d():
+    st, _ = states.random_state(4, 3, 1, 2, np.float64, seed=231)
+    from cabanapic_amd import CpicError
+    with sim_from_state(st) as sim:
+        with pytest.raises(CpicError):
+            sim.set_field_solver("es_1d")
+        sim.set_field_solver("em")

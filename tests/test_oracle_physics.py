@@ -29,6 +29,10 @@ exercised there).  These checks close that gap:
   dispersion sin(theta/2) = p sin(k d/2); the Gauss residual
   div E - rho / eps0 is invariant under full steps with particles.
 * energies (src/fields.h:556-615): brute-force sum in long double.
+* electrostatic 1D solver (src/fields.h:467-545): with no ghost current,
+  E moves by exactly -dt_eps0 J on every voxel; the Gauss residual is
+  invariant over full ES steps with particles; the all-voxel energy is a
+  brute-force sum.
 """
 import numpy as np
 import pytest
@@ -527,3 +531,73 @@ def test_energies_brute_force(dtype):
     eps = np.finfo(dtype).eps
     assert abs(er - float(ew)) <= 200 * eps * float(ew)
     assert abs(br - float(bw)) <= 200 * eps * float(bw)
+
+
+# ----------------------------------------------------------------------------
+# electrostatic 1D solver
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_es_advance_e_moves_e_by_minus_dt_j(dtype):
+    """With J zero outside the interior the fold changes nothing, and every
+    voxel (ghosts included) gets E + (-dt_eps0 J) in real_t; B and J stay."""
+    st = _single(7, 1, 1, 1, dtype)
+    rng = np.random.default_rng(21)
+    st.fields[:] = rng.uniform(-1, 1, st.fields.shape).astype(dtype)
+    inner = _grid(st, np.ones(st.ng, bool)).copy()
+    inner[:] = False
+    inner[1:-1, 1:-1, 1:-1] = True
+    for m in (6, 7, 8):
+        st.fields[m][~inner.ravel()] = 0
+    before = st.fields.copy()
+    cj = dtype(0.37)
+    st.es_advance_e(cj)
+    for m in range(3):
+        assert np.array_equal(st.fields[m], before[m] + (-cj) * before[6 + m]), m
+    assert np.array_equal(st.fields[3:], before[3:])
+
+
+def test_es_gauss_residual_invariant_1d():
+    """Full electrostatic steps (load_interpolator, push, unload, ES
+    advance_e) on a periodic line: div E - rho / eps0 at every node is
+    unchanged, which fixes the sign and the dt_eps0 factor of the update
+    and the fold it relies on."""
+    nx = 9
+    rng = np.random.default_rng(22)
+    n = 40
+    st = _single(nx, 1, 1, n, F64)
+    st.cell[:] = _voxel(st, rng.integers(1, nx + 1, n), 1, 1)
+    for m in ("x", "y", "z"):
+        st.p[m][:] = rng.uniform(-1, 1, n)
+    for m in ("ux", "uy", "uz"):
+        st.p[m][:] = rng.uniform(-0.8, 0.8, n)
+    st.p["w"][:] = rng.uniform(0.5, 1.5, n)
+    k = states.step_consts(F64)
+    k["dt_eps0"] = k["dt"] / 2.0  # eps0 = 2, so dt_eps0 and dt differ
+    dx, dy, dz, dt = k["dx"], k["dy"], k["dz"], k["dt"]
+    eps0 = dt / k["dt_eps0"]
+
+    def residual():
+        f = st.fields
+        return (_node_divergence(st, f[0], f[1], f[2], dx, dy, dz)
+                - _node_charge(st, k["qsp"]) / (dx * dy * dz * eps0))
+
+    r0 = residual()
+    for _ in range(20):
+        _push_once(st, k)
+        st.unload(dx, dy, dz, dt)
+        st.es_advance_e(k["dt_eps0"])
+    r1 = residual()
+    ediv = np.abs(_node_divergence(st, *st.fields[:3], dx, dy, dz)).max()
+    assert ediv > 1.0
+    assert np.abs(r1 - r0).max() < 1e-11 * ediv, np.abs(r1 - r0).max()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_es_energy_brute_force(dtype):
+    st = _single(6, 1, 1, 1, dtype)
+    st.fields[:] = np.random.default_rng(23).uniform(-2, 2, st.fields.shape).astype(dtype)
+    e64, er = st.es_energy()
+    ld = np.longdouble
+    ew = ld(0.5) * sum((st.fields[m].astype(ld) ** 2).sum() for m in range(3))
+    assert abs(ld(e64) - ew) <= 1e-14 * ew
+    assert abs(er - float(ew)) <= 200 * np.finfo(dtype).eps * float(ew)
