@@ -32,11 +32,14 @@ inline bool within_tolerance(double a, double b, double tol, double* err) {
   return *err < tol;
 }
 
-// Returns true when every compared line of file_b matches file_a.  When
-// err_file is non-empty the per-line errors (in percent) are written there.
+// Returns true when every compared line of file_b matches file_a.  The
+// parameters are those of the reference's compare_energies
+// (tests/energy_comparison/compare_energies.h:135-145); when write_err_output
+// is nonzero the per-line errors (in percent) are written to err_file.
 inline bool compare_energies(const std::string& file_a, const std::string& file_b, double tol,
-                             unsigned mask = 0xffffu, FIELD_ENUM mode = Individual,
-                             const std::string& err_file = "", int skip = 0, int count = -1) {
+                             unsigned short mask = 0xffffu, FIELD_ENUM mode = Individual,
+                             int write_err_output = 0, const std::string& err_file = "err.out",
+                             int skip = 0, int count = -1) {
   std::ifstream fa(file_a), fb(file_b);
   if (!fa.is_open()) {
     std::cerr << "Unable to open file " << file_a << std::endl;
@@ -92,7 +95,7 @@ inline bool compare_energies(const std::string& file_a, const std::string& file_
   }
   std::cout << "Max found err was " << worst * 100 << "% (" << worst_a << " vs " << worst_b
             << ") on line " << worst_line << " (Threshold: " << tol * 100 << "%)" << std::endl;
-  if (!err_file.empty()) {
+  if (write_err_output) {
     std::ofstream out(err_file);
     for (size_t i = 0; i < errs.size(); ++i) out << i + 1 << " " << errs[i] * 100.0 << "\n";
   }

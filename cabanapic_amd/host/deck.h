@@ -14,6 +14,14 @@
 
 #include "pic_types.h"
 
+// src/input/deck.h:416-417 (decks stringify build-time paths with them)
+#ifndef STRINGIFY
+#define STRINGIFY(s) #s
+#endif
+#ifndef EXPAND
+#define EXPAND(s) STRINGIFY(s)
+#endif
+
 // Hook run after the simulation loop (src/input/deck.h:15-21).
 class Run_Finalizer {
  public:

@@ -22,10 +22,10 @@ class Gold_Check : public Run_Finalizer {
     const double margin = 0.10;
     const int skip = 3581, lines = 1300;
     const bool e_ok = test_utils::compare_energies(gold, "energies.txt", margin, 0b100u,
-                                                   test_utils::Sum, "e.out", skip, lines);
+                                                   test_utils::Sum, 1, "e.out", skip, lines);
     std::cout << "E Test Pass: " << e_ok << std::endl;
     const bool b_ok = test_utils::compare_energies(gold, "energies.txt", margin, 0b1000u,
-                                                   test_utils::Sum, "b.out", skip, lines);
+                                                   test_utils::Sum, 1, "b.out", skip, lines);
     std::cout << "B Test Pass: " << b_ok << std::endl;
     if (!e_ok || !b_ok) std::exit(1);
   }

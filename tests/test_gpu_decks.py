@@ -8,6 +8,11 @@
   E rule is order sensitive (see FLOAT_E_ENVELOPE).
 * deck-override smoke test (tests/decks): a deck with its own particle
   initializer runs 30 steps and exits 0.
+* the reference's own deck sources, compiled unchanged against the deck
+  interface where the reference tree is present (the executables travel
+  with the tree): its regression test tests/energy_comparison/2stream-em.cxx
+  with its own gold check, and decks/2stream-short.cxx, custom_init.cxx and
+  dioctron_3d.cxx, which run to the end with finite energies.
 * the electrostatic build (the reference's SOLVER_TYPE=ES): the 1D deck's
   energies.txt holds step, time and E, and E follows the oracle's ES stage
   loop from the deck's own initial state.
@@ -121,3 +126,54 @@ def test_es_solver_deck(real, tmp_path):
     # 2.1e-3 of the largest energy over the 30 steps (1.3e-12 in double).
     rel = np.abs(h[:, 2] - e) / np.abs(e).max()
     assert rel.max() < (5e-3 if real == "float" else 2e-5), rel.max()
+
+
+def _ref_exe(name, real):
+    exe = os.path.join(host_driver.HOST, "build", f"cbnpic_{name}_{real}")
+    if not os.path.exists(exe):
+        pytest.skip("the reference's decks are built only where the reference tree is present")
+    return exe
+
+
+@pytest.mark.parametrize("real", ["float", "double"])
+def test_reference_regression_test_unchanged(real, tmp_path):
+    """The reference's regression test source, built unchanged: its own
+    finalizer checks the 6000-step history against the gold file with its
+    10 % rule.  Double passes both rules and matches the gold to its printed
+    digits; float passes B and keeps E inside the order-only envelope."""
+    exe = _ref_exe("ref_2stream_em", real)
+    print(f"running {os.path.basename(exe)}", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    err_e, err_b = _rule_errors(r.stdout)
+    assert "B Test Pass: 1" in r.stdout
+    h = np.loadtxt(tmp_path / "energies.txt")
+    g = energy_rule.gold(real)
+    assert h.shape == g.shape
+    if real == "double":
+        assert r.returncode == 0 and "E Test Pass: 1" in r.stdout, r.stdout[-2000:]
+        assert (np.abs(h[:, 2:4] - g[:, 2:4]) / g[:, 2:4]).max() < 1e-5
+    else:
+        assert r.returncode in (0, 1) and err_e < FLOAT_E_ENVELOPE, (err_e, err_b)
+
+
+@pytest.mark.parametrize("name,steps", [("ref_custom_init", 30), ("ref_dioctron_3d", 20000)])
+def test_reference_decks_run_unchanged(name, steps, tmp_path):
+    exe = _ref_exe(name, "double")
+    print(f"running {os.path.basename(exe)}", flush=True)
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    h = np.loadtxt(tmp_path / "energies.txt")
+    assert h.shape == (steps, 4) and np.all(np.isfinite(h))
+
+
+def test_reference_deck_with_ghost_particles_is_rejected(tmp_path):
+    """decks/2stream-short.cxx is 1D in x (32 x 1 x 1 cells) but keeps the
+    default initializer, which spreads the beams along y
+    (src/input/deck.h:79-158): particle 100 onwards land in ghost voxels
+    (y = 2 of a one-cell-high grid).  The reference pushes them on the zero
+    ghost interpolators; this build refuses particles outside the interior
+    at upload, with an error naming the first one, instead of running."""
+    exe = _ref_exe("ref_2stream_short", "double")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1
+    assert "particle 100 has non-interior cell" in r.stderr, r.stderr[-2000:]
