@@ -59,6 +59,35 @@ void dump_energy_line(int step, real_t time, real_t e, real_t b) {
 #endif
 }
 
+// dump_particles, src/helpers.h:31-70: the physical x position and ux of
+// every particle, "%e  %e " each, then a newline
+void write_particles(FILE* fp, size_t n, const real_t* px, const real_t* ux,
+                     const int32_t* cell, real_t xmin, real_t dx, size_t nx, size_t ny,
+                     size_t ng) {
+  for (size_t p = 0; p < n; ++p) {
+    size_t ix, iy, iz;
+    const int ii = cell[p];
+    RANK_TO_INDEX(ii, ix, iy, iz, nx + 2 * ng, ny + 2 * ng);
+    (void)iy;
+    (void)iz;
+    const real_t x = xmin + (ix - 1 + (px[p] + 1.0) * 0.5) * dx;
+    const real_t v = ux[p];
+    std::fprintf(fp, "%e  %e ", (double)x, (double)v);
+  }
+  std::fprintf(fp, "\n");
+}
+
+// Field_Solver::dump_fields, src/fields.h:318-350: ex along the line y = z = 1
+void write_fields(FILE* fp, const real_t* ex, real_t xmin, real_t dx, size_t nx, size_t ny,
+                  size_t nz, size_t ng) {
+  for (size_t i = 1; i < nx + 1; i++) {
+    const real_t x = xmin + (i - 0.5) * dx;
+    const size_t ii = VOXEL(i, 1, 1, nx, ny, nz, ng);
+    std::fprintf(fp, "%e %e\n", (double)x, (double)ex[ii]);
+  }
+  std::fprintf(fp, "\n\n");
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -66,9 +95,12 @@ int main(int argc, char** argv) {
   long chunk = 200;
   const char* dump = nullptr;
   bool dump_particles = true;
+  bool dumps = false;
   for (int i = 1; i < argc; ++i) {
     if (!std::strcmp(argv[i], "--stages")) {
       stages = true;
+    } else if (!std::strcmp(argv[i], "--dumps")) {
+      dumps = true;
     } else if (!std::strcmp(argv[i], "--chunk") && i + 1 < argc) {
       chunk = std::atol(argv[++i]);
       if (chunk < 1) chunk = 1;
@@ -78,7 +110,9 @@ int main(int argc, char** argv) {
       dump = argv[++i];
       dump_particles = false;
     } else {
-      std::fprintf(stderr, "usage: %s [--stages] [--chunk N] [--dump-initial|--dump-constants FILE]\n",
+      std::fprintf(stderr,
+                   "usage: %s [--stages] [--chunk N] [--dumps] "
+                   "[--dump-initial|--dump-constants FILE]\n",
                    argv[0]);
       return 2;
     }
@@ -164,6 +198,42 @@ int main(int argc, char** argv) {
     return 1;
   }
 
+  // the reference's per-step dumps (example/example.cpp:59-60,130-132,
+  // 273-277): partloc (particles) and ex1d (ex along x), with --dumps
+  FILE* fptr = nullptr;
+  FILE* fpfd = nullptr;
+  std::vector<real_t> dump_p[7];
+  std::vector<int32_t> dump_cell;
+  std::vector<real_t> dump_f;
+  if (dumps) {
+    fptr = std::fopen("partloc", "w");
+    fpfd = std::fopen("ex1d", "w");
+    if (!fptr || !fpfd) {
+      std::fprintf(stderr, "cbnpic: cannot open partloc / ex1d\n");
+      return 1;
+    }
+    std::fprintf(fptr, "#step=0\n0 ");
+    write_particles(fptr, num_particles, particles.data(PositionX), particles.data(VelocityX),
+                    particles.cells(), 0, dx, nx, ny, num_ghosts);
+    chunk = 1;
+    for (auto& v : dump_p) v.resize(num_particles);
+    dump_cell.resize(num_particles);
+    dump_f.resize(9 * (size_t)(nx + 2) * (ny + 2) * (nz + 2));
+  }
+  auto dump_step = [&](cpic_context* ctx, int step) {
+    if (!dumps) return;
+    CPIC_CALL(cpic_download_fields(ctx, dump_f.data()));
+    std::fprintf(fpfd, "#step=%d\n", step);
+    write_fields(fpfd, dump_f.data(), 0, dx, nx, ny, nz, num_ghosts);
+    CPIC_CALL(cpic_download_particles(ctx, (int64_t)num_particles, dump_p[0].data(),
+                                      dump_p[1].data(), dump_p[2].data(), dump_p[3].data(),
+                                      dump_p[4].data(), dump_p[5].data(), dump_p[6].data(),
+                                      dump_cell.data()));
+    std::fprintf(fptr, "#step=%d\n%e ", step, (double)(step * dt));
+    write_particles(fptr, num_particles, dump_p[0].data(), dump_p[3].data(), dump_cell.data(), 0,
+                    dx, nx, ny, num_ghosts);
+  };
+
   cpic_config cfg{};
   cfg.real_size = (int32_t)sizeof(real_t);
   cfg.num_ghosts = num_ghosts;
@@ -202,6 +272,7 @@ int main(int argc, char** argv) {
       double e = 0, b = 0;
       CPIC_CALL(cpic_energies(ctx, &e, &b));
       dump_energy_line(step, step * dt, (real_t)e, (real_t)b);
+      dump_step(ctx, step);
     }
   } else {
     cpic_step_params sp{};
@@ -233,8 +304,11 @@ int main(int argc, char** argv) {
         const int step = first + k;
         dump_energy_line(step, step * dt, (real_t)hist[3 * k], (real_t)hist[3 * k + 1]);
       }
+      if (n == 1) dump_step(ctx, first);
     }
   }
+  if (fptr) std::fclose(fptr);
+  if (fpfd) std::fclose(fpfd);
   int32_t flags = 0;
   CPIC_CALL(cpic_device_errors(ctx, &flags));
   CPIC_CALL(cpic_destroy(ctx));

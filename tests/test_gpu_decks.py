@@ -177,3 +177,43 @@ def test_reference_deck_with_ghost_particles_is_rejected(tmp_path):
     r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 1
     assert "particle 100 has non-interior cell" in r.stderr, r.stderr[-2000:]
+
+
+def _expected_particle_line(p, dx, nx, ny, real):
+    """One partloc line in the reference's format (src/helpers.h:31-70),
+    computed with its arithmetic: x = xmin + (ix - 1 + (px + 1.0) * 0.5) * dx
+    in double from real_t inputs, rounded to real_t."""
+    R = np.float32 if real == "float" else np.float64
+    X, Y = nx + 2, ny + 2
+    ix = p["cell"] % X
+    x = (ix - 1 + (p["x"].astype(np.float64) + 1.0) * 0.5) * np.float64(R(dx))
+    x = x.astype(R)
+    return "".join(f"{float(a):e}  {float(b):e} " for a, b in zip(x, p["ux"]))
+
+
+@pytest.mark.parametrize("real", ["float", "double"])
+def test_reference_dumps(real, tmp_path):
+    """--dumps writes the reference's partloc and ex1d files
+    (example/example.cpp:59-60,130-132,273-277): a step-0 particle line
+    equal to the reference's formula applied to the deck's initial state,
+    then one block per step with every particle and the nx ex samples."""
+    r = run("beam_1d_custom_init", real, tmp_path, "--dumps")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    k, p = host_driver.dump("beam_1d_custom_init", real, str(tmp_path / "init.bin"))
+    n = p["cell"].size
+    lines = (tmp_path / "partloc").read_text().split("\n")
+    assert lines[0] == "#step=0"
+    assert lines[1] == "0 " + _expected_particle_line(p, k["dx"], 32, 1, real)
+    for step in range(1, 31):
+        head, body = lines[2 * step], lines[2 * step + 1]
+        assert head == f"#step={step}"
+        vals = body.split()
+        assert len(vals) == 1 + 2 * n and float(vals[0]) == pytest.approx(step * k["dt"], rel=1e-6)
+        assert np.all(np.isfinite(np.array(vals, float)))
+    blocks = (tmp_path / "ex1d").read_text().split("#step=")[1:]
+    assert len(blocks) == 30
+    for step, b in enumerate(blocks, 1):
+        rows = [l.split() for l in b.strip().split("\n")[1:]]
+        assert b.startswith(f"{step}\n") and len(rows) == 32
+        xs = np.array([float(r_[0]) for r_ in rows])
+        assert np.allclose(xs, (np.arange(1, 33) - 0.5) * k["dx"], rtol=1e-6)
