@@ -107,6 +107,19 @@ def run_tangent(deck, steps, warmup):
     return t1 - t0
 
 
+def differentiation_config_costs(steps):
+    """Forward step and tangent-linear step on the differentiation
+    configuration (3D two-stream, 32^3 cells, 2 x 16 ppc, FP32 primal, FP64
+    tangent): ms per step of each."""
+    from cabanapic_amd import decks
+    d = decks.two_stream_3d(np.float32, n=32, nppc=32, steps=steps)
+    t_fwd, _, _, errs = run_gpu(d, steps, 10, 0, True)
+    if errs:
+        raise SystemExit(f"device error flags {errs}")
+    t_tan = run_tangent(d, steps, 10)
+    return 1e3 * t_fwd / steps, 1e3 * t_tan / steps
+
+
 def cpu_baseline(deck, nsteps, nthreads):
     """The oracle's OpenMP step on the full workload for nsteps steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -214,6 +227,12 @@ def main():
               file=sys.stderr, flush=True)
         extra.update(tangent_ms_per_step=round(tms, 3),
                      tangent_over_forward=round(tms / ms_step, 3))
+        f4, t4 = differentiation_config_costs(100)
+        print(f"# differentiation config 32^3 x 32: forward {f4:.3f} ms/step, tangent "
+              f"{t4:.3f} ms/step ({t4 / f4:.2f} x)", file=sys.stderr, flush=True)
+        extra.update(diffconfig_forward_ms_per_step=round(f4, 4),
+                     diffconfig_tangent_ms_per_step=round(t4, 4),
+                     diffconfig_tangent_over_forward=round(t4 / f4, 3))
 
     cpu = None
     if args.cpu_steps > 0:

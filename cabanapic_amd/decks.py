@@ -153,16 +153,19 @@ def weibel_3d(dtype=np.float32, n=128, nppc=32, steps=500, seed=11):
 
 
 def two_stream_3d(dtype=np.float32, n=64, nppc=64, steps=200, seed=7, v0=0.2, uth=0.01,
-                  amp=1e-3):
+                  amp=1e-3, length=None):
     """3D two-stream: counter-streaming beams +-v0 along x (nppc/2 each),
-    Maxwellian spread uth per component, mode-1 drift perturbation."""
+    Maxwellian spread uth per component, mode-1 drift perturbation.  The box
+    edge is `length`, by default 2 pi v0 / 0.5 (derivatives with respect to
+    v0 hold it fixed)."""
     d = Deck(dtype)
     R = d.R
     d.nx = d.ny = d.nz = n
     d.nppc = nppc
     d.num_steps = steps
     d.v0 = R(v0)
-    L = R(2.0 * math.pi * v0 / 0.5)  # fastest-growing two-stream wavelength scale
+    # fastest-growing two-stream wavelength scale
+    L = R(2.0 * math.pi * v0 / 0.5) if length is None else R(length)
     d.len_x_global = d.len_y_global = d.len_z_global = L
     d.dt = R(F64(0.99) * F64(d.courant_length(L, L, L, n, n, n)) / F64(d.c))
     d.n0 = R(1.0)
@@ -173,7 +176,32 @@ def two_stream_3d(dtype=np.float32, n=64, nppc=64, steps=200, seed=7, v0=0.2, ut
     return d.derive_params()
 
 
+def two_stream_1d(dtype=np.float64, nx=32, nppc_per_beam=32, steps=1000, seed=1234, v0=0.2,
+                  amp=1e-3):
+    """1D-periodic two-stream in x (the reference's 2stream-short case with
+    the beams spread along x): nx x 1 x 1 cells, Lx = 2 pi 0.2 / 0.5, two
+    cold counter-streaming electron beams at +-v0 c along x (nppc_per_beam
+    each, alternating), offsets uniform-random from the seed, drift
+    modulated by 1 + amp sin(2 pi x / Lx), immobile ions, EM solver,
+    dt = 0.99 Courant."""
+    d = Deck(dtype)
+    R = d.R
+    d.nx, d.ny, d.nz = nx, 1, 1
+    d.nppc = 2 * nppc_per_beam
+    d.num_steps = steps
+    d.v0 = R(v0)
+    d.len_x_global = R(2.0 * math.pi * v0 / 0.5)
+    d.dt = R(F64(0.99) * F64(d.courant_length(d.len_x_global, d.len_y_global, d.len_z_global,
+                                                nx, 1, 1)) / F64(d.c))
+    d.n0 = R(1.0)
+    d.name = f"two_stream_1d_{nx}x{d.nppc}"
+    d.plasma = dict(seed=seed, drift=(v0 / math.sqrt(1.0 - v0 * v0), 0.0, 0.0), pert_amp=amp,
+                    pert_mode=1, counter_streaming=True)
+    return d.derive_params()
+
+
 CONFIGS = {
+    "twostream1d": two_stream_1d,
     "weibel128": weibel_3d,
     "twostream64": two_stream_3d,
 }

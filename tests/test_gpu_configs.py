@@ -1,0 +1,262 @@
+"""Parity on the benchmark configurations at their full sizes.
+
+Each configuration is loaded by its deck and run by the fused step
+(cpic_step) through the C-ABI.  At chosen steps the state before the step is
+downloaded, the step runs on the device, and the oracle pushes the same
+particles on the interpolators of the same fields: after move_p every
+particle (cell index, offsets, momenta, weight) must be bit-identical and the
+particle count conserved.  Between checks the device runs on its own, so the
+checked steps include column-packing steps (the first after the upload and
+every 32nd) and steps after long runs, where the columns hold holes.
+
+Configurations (cells, particles per cell):
+  1D two-stream      32 x 1 x 1, 2 x 32, 1000 steps, every step checked
+  3D two-stream      64^3, 64 (16,777,216 particles), 200 steps
+  Weibel             128^3, 32 (67,108,864 particles), the benchmark workload
+The 3D two-stream and Weibel runs also compare their FP32 and FP64 field
+energy histories.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+from cabanapic_amd import Simulation, decks
+
+pytestmark = pytest.mark.gpu
+
+MEMBERS = ("x", "y", "z", "ux", "uy", "uz", "w")
+NTHREADS = 16
+
+
+def _name(dtype):
+    return "f32" if dtype == np.float32 else "f64"
+
+
+def _bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view({4: np.uint32, 8: np.uint64}[a.dtype.itemsize]).astype(np.uint64)
+
+
+def _keys(p):
+    """One 64-bit key per particle: the cell index in the top 24 bits and a
+    40-bit hash of the bit patterns of all members below it.  Two particle
+    sets are the same multiset (up to a hash collision, probability about
+    count^2 / 2^41 per cell) exactly when their sorted keys are equal."""
+    h = np.zeros(p["cell"].size, np.uint64)
+    mult = np.uint64(0x9E3779B97F4A7C15)
+    for m in MEMBERS:
+        h ^= _bits(p[m])
+        h *= mult
+        h ^= h >> np.uint64(29)
+    return (p["cell"].astype(np.uint64) << np.uint64(40)) | (h >> np.uint64(24))
+
+
+def assert_same_particles(got, ref, what):
+    assert got["cell"].size == ref["cell"].size, f"{what}: particle count changed"
+    kg, kr = _keys(got), _keys(ref)
+    if np.array_equal(np.sort(kg), np.sort(kr)):
+        return
+    # name the members that differ, in the canonical order of the keys
+    og, orf = np.argsort(kg, kind="stable"), np.argsort(kr, kind="stable")
+    bad = []
+    for m in ("cell",) + MEMBERS:
+        n = int(np.count_nonzero(_bits(got[m][og]) != _bits(ref[m][orf])))
+        if n:
+            bad.append(f"{m} ({n} of {og.size})")
+    raise AssertionError(f"{what}: particles differ from the oracle's: {', '.join(bad)}")
+
+
+def oracle_push(deck, fields, parts):
+    """The oracle's push of a copy of `parts` on the interpolators of
+    `fields`."""
+    st = oracle_lib.OracleState(deck.nx, deck.ny, deck.nz, parts["cell"].size, deck.R)
+    st.fields[:] = fields
+    for m in MEMBERS:
+        st.p[m] = np.array(parts[m], deck.R)
+    st.cell = np.array(parts["cell"], np.int32)
+    st.load_interpolator()
+    st.clear_accumulator()
+    k = deck.driver_constants()
+    st.push(k["qdt_2mc"], k["cdt_dx"], k["cdt_dy"], k["cdt_dz"], k["qsp"], nthreads=NTHREADS)
+    out = dict(st.p)
+    out["cell"] = st.cell
+    return out
+
+
+def checked_step(sim, deck, sp, what):
+    """One fused step on the device, its push checked against the oracle's
+    on the identical pre-step state; returns the step's energy row."""
+    f0 = sim.download_fields()
+    p0 = sim.download_particles()
+    h = sim.step(sp, 1)
+    got = sim.download_particles()
+    assert sim.device_errors() == 0, what
+    ref = oracle_push(deck, f0, p0)
+    del f0, p0
+    assert_same_particles(got, ref, what)
+    return h[0]
+
+
+def load(deck):
+    parts = deck.load_particles()
+    sim = Simulation(deck.nx, deck.ny, deck.nz, deck.num_particles, deck.R)
+    sim.upload_particles(*[parts[m] for m in MEMBERS], parts["cell"])
+    return sim
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=_name)
+def test_two_stream_1d_every_step(dtype):
+    """32 x 1 x 1 cells, two cold beams of 32 particles per cell at +-0.2 c,
+    seed 1234, 1000 steps: every push is bit-identical to the oracle's on the
+    same input, and the device's energy history follows the oracle's own
+    1000-step run (in double to the reordered-sum level until the
+    instability amplifies it; the growth phase within 1e-3 in both)."""
+    deck = decks.two_stream_1d(dtype)
+    sp = deck.step_params(energy_interval=1)
+    hist = np.zeros((deck.num_steps, 3))
+    with load(deck) as sim:
+        for s in range(deck.num_steps):
+            hist[s] = checked_step(sim, deck, sp, f"step {s + 1}")
+    # the oracle's own run of the whole deck
+    parts = deck.load_particles()
+    st = oracle_lib.OracleState(deck.nx, deck.ny, deck.nz, deck.num_particles, dtype)
+    for m in MEMBERS:
+        st.p[m] = parts[m]
+    st.cell = parts["cell"]
+    k = deck.driver_constants()
+    consts = [k[n] for n in ("qdt_2mc", "cdt_dx", "cdt_dy", "cdt_dz", "qsp", "dx", "dy", "dz",
+                             "dt", "px", "py", "pz", "dt_eps0", "me", "c")]
+    h, _ = st.run(consts, deck.num_steps)
+    tot_g = hist[:, 0] + hist[:, 1]
+    tot_o = h[:, 0] + h[:, 1]
+    rel = np.abs(tot_g - tot_o) / tot_o
+    print(f"1D two-stream {_name(dtype)}: field energy rel. difference to the oracle run, "
+          f"steps 1-200 {rel[:200].max():.2e}, 1-400 {rel[:400].max():.2e}, "
+          f"all {rel.max():.2e}; final E {tot_g[-1]:.4e}")
+    assert rel[:200].max() < (1e-4 if dtype == np.float32 else 1e-10)
+    assert rel[:400].max() < 1e-3
+    # the instability grows: field energy rises by orders of magnitude
+    assert tot_g[-1] > 100 * tot_g[0]
+
+
+def _run_checked(deck, checks, total):
+    """Runs `total` steps with the pushes of the steps in `checks` (1-based)
+    checked; returns the energy history."""
+    sp = deck.step_params(energy_interval=1)
+    hist = np.zeros((total, 3))
+    s = 0
+    with load(deck) as sim:
+        n0 = sim.num_particles()
+        for c in sorted(checks):
+            if c - 1 > s:
+                hist[s:c - 1] = sim.step(sp, c - 1 - s)
+                s = c - 1
+            print(f"{deck.name} {_name(deck.R)}: checking step {c}", flush=True)
+            hist[s] = checked_step(sim, deck, sp, f"{deck.name} step {c}")
+            s = c
+        if total > s:
+            hist[s:] = sim.step(sp, total - s)
+        assert sim.num_particles() == n0
+        assert sim.device_errors() == 0
+    return hist
+
+
+@pytest.mark.timeout(600)
+def test_two_stream_3d_full_size():
+    """64^3 cells x 64 ppc (16.8M particles), 200 steps in FP32 and FP64:
+    pushes bit-identical to the oracle at steps 1, 2, 33 (a packing step
+    after holes) and 200; FP32 field energy within 1e-3 of FP64."""
+    h = {}
+    for dtype in (np.float32, np.float64):
+        deck = decks.two_stream_3d(dtype)
+        h[dtype] = _run_checked(deck, (1, 2, 33, 200), deck.num_steps)
+    e32 = h[np.float32][:, 0] + h[np.float32][:, 1]
+    e64 = h[np.float64][:, 0] + h[np.float64][:, 1]
+    rel = np.abs(e32 - e64) / e64
+    print(f"3D two-stream: FP32 vs FP64 field energy, max rel. difference over 200 steps "
+          f"{rel.max():.2e}")
+    assert rel.max() < 1e-3
+
+
+@pytest.mark.timeout(900)
+def test_weibel_full_size():
+    """128^3 cells x 32 ppc (67.1M particles), the benchmark workload: FP32
+    pushes bit-identical to the oracle at steps 1, 2 and 40 (after a
+    packing step and holes); FP64 at steps 1 and 2; over 100 steps FP32
+    and FP64 field energies agree within 1e-3."""
+    h32 = _run_checked(decks.weibel_3d(np.float32), (1, 2, 40), 100)
+    h64 = _run_checked(decks.weibel_3d(np.float64), (1, 2), 100)
+    e32 = h32[:, 0] + h32[:, 1]
+    e64 = h64[:, 0] + h64[:, 1]
+    rel = np.abs(e32 - e64) / e64
+    print(f"Weibel: FP32 vs FP64 field energy, max rel. difference over 100 steps "
+          f"{rel.max():.2e}")
+    assert rel.max() < 1e-3
+
+
+def _loader_tangent(factory, name, value, rel=1e-6):
+    """d(ux)/d(parameter) of the deck's FP64 particle loader by central
+    differences of the loader itself (ux is linear in the drift and in the
+    perturbation amplitude, so only rounding enters); the FP32 deck loads
+    the same values rounded to float."""
+    lo = factory(np.float64, **{name: value * (1 - rel)}).load_particles()
+    hi = factory(np.float64, **{name: value * (1 + rel)}).load_particles()
+    for m in ("x", "y", "z", "uy", "uz", "w", "cell"):
+        assert np.array_equal(lo[m], hi[m]), m
+    t = np.zeros((6, lo["cell"].size))
+    t[3] = (hi["ux"] - lo["ux"]) / (2 * rel * value)
+    return t
+
+
+def _final_e_energy(deck, parts):
+    sp = deck.step_params(energy_interval=deck.num_steps)
+    with Simulation(deck.nx, deck.ny, deck.nz, deck.num_particles, deck.R) as sim:
+        sim.upload_particles(*[parts[m] for m in MEMBERS], parts["cell"])
+        h = sim.step(sp, deck.num_steps)
+        assert sim.device_errors() == 0
+    return h[-1, 0]
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("param,value", [("v0", 0.2), ("amp", 1e-3)])
+def test_differentiation_config(param, value):
+    """The differentiation configuration: 3D two-stream, 32^3 cells, 16
+    particles per cell per beam (1,048,576), drift v0 = 0.2 c, perturbation
+    amplitude 1e-3, 500 steps.  The derivative of the final E-field energy
+    with respect to v0 and to the amplitude from the tangent-linear step
+    (FP64 primal, FP64 tangent) equals central differences of the fused FP64
+    step with eps = 1e-4 relative; the FP32-primal tangent (FP64 tangent
+    accumulation) follows the FP64 one."""
+    from cabanapic_amd import TangentSimulation
+
+    def factory(dtype, **kw):
+        args = dict(n=32, nppc=32, steps=500, v0=0.2, amp=1e-3, length=2.0 * np.pi * 0.2 / 0.5)
+        args.update(kw)
+        return decks.two_stream_3d(dtype, **args)
+
+    tan = _loader_tangent(factory, param, value)
+    grads = {}
+    for dtype in (np.float64, np.float32):
+        deck = factory(dtype)
+        parts = deck.load_particles()
+        with TangentSimulation(deck.nx, deck.ny, deck.nz, deck.num_particles, dtype) as ts:
+            ts.upload_particles(*[parts[m] for m in MEMBERS], parts["cell"], tan)
+            hist = ts.step(deck.step_params(energy_interval=1), deck.num_steps)
+            assert ts.primal.device_errors() == 0
+        grads[dtype] = (hist[-1, 0], hist[-1, 2])
+    e64, g64 = grads[np.float64]
+    e32, g32 = grads[np.float32]
+    eps = 1e-4
+    jp = _final_e_energy(factory(np.float64, **{param: value * (1 + eps)}),
+                         factory(np.float64, **{param: value * (1 + eps)}).load_particles())
+    jm = _final_e_energy(factory(np.float64, **{param: value * (1 - eps)}),
+                         factory(np.float64, **{param: value * (1 - eps)}).load_particles())
+    fd = (jp - jm) / (2 * eps * value)
+    rel = abs(g64 - fd) / abs(fd)
+    rel32 = abs(g32 - g64) / abs(g64)
+    print(f"d(E energy at step 500)/d({param}): tangent FP64 {g64:.6e}, central FD {fd:.6e} "
+          f"(rel. difference {rel:.2e}); tangent on the FP32 primal {g32:.6e} "
+          f"(rel. difference to FP64 {rel32:.2e}); E energy {e64:.6e} (FP32 {e32:.6e})")
+    assert np.isfinite(g64) and fd != 0
+    assert rel < 1e-3
+    assert rel32 < 1e-1

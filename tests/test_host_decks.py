@@ -32,7 +32,8 @@ def test_regression_deck_initial_state_matches_oracle(real, tmp_path):
 
 @pytest.mark.parametrize("real", ["float", "double"])
 @pytest.mark.parametrize("deck,factory", [("weibel_3d", decks.weibel_3d),
-                                          ("two_stream_3d", decks.two_stream_3d)])
+                                          ("two_stream_3d", decks.two_stream_3d),
+                                          ("two_stream_1d", decks.two_stream_1d)])
 def test_python_and_cpp_decks_agree(real, deck, factory, tmp_path):
     dtype = np.float32 if real == "float" else np.float64
     k, _ = host_driver.dump(deck, real, str(tmp_path / "c.bin"), particles=False)
