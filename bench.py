@@ -59,6 +59,10 @@ def run_gpu(deck, steps, warmup, compact_interval, history):
     sim.upload_particles(*[parts[k] for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
                          parts["cell"])
     del parts
+    fields = deck.load_fields()
+    if fields is not None:
+        sim.upload_fields(fields)
+        del fields
     sp = deck.step_params(energy_interval=1, compact_interval=compact_interval)
     if warmup:
         sim.step(sp, warmup)
@@ -156,6 +160,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--tangent-steps", type=int, default=5,
                     help="timed steps of the tangent-linear step (0 = skip)")
+    ap.add_argument("--column-steps", type=int, default=20,
+                    help="timed FP32 steps of the 192x192x64 x 32 ppc electron-column "
+                         "configuration, reported beside the metric (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (torch), the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -233,6 +240,18 @@ def main():
         extra.update(diffconfig_forward_ms_per_step=round(f4, 4),
                      diffconfig_tangent_ms_per_step=round(t4, 4),
                      diffconfig_tangent_over_forward=round(t4 / f4, 3))
+
+    if args.column_steps > 0:
+        dc = decks.dioctron_3d(np.float32)
+        tc, _, stc, errc = run_gpu(dc, args.column_steps, args.warmup, args.compact_interval,
+                                   True)
+        if errc:
+            raise SystemExit(f"device error flags {errc}")
+        vc = dc.num_particles * args.column_steps / tc
+        print(f"# {dc.name}: {1e3 * tc / args.column_steps:.3f} ms/step, {vc:.4g} "
+              "particle-steps/s", file=sys.stderr, flush=True)
+        extra.update(column_workload=dc.name, column_particles=dc.num_particles,
+                     column_value=vc, column_ms_per_step=round(1e3 * tc / args.column_steps, 4))
 
     cpu = None
     if args.cpu_steps > 0:

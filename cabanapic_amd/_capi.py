@@ -54,7 +54,11 @@ class PlasmaSpec(ctypes.Structure):
                 ("nz", ctypes.c_int64), ("nppc", ctypes.c_int64), ("weight", ctypes.c_double),
                 ("uth", ctypes.c_double * 3), ("drift", ctypes.c_double * 3),
                 ("pert_amp", ctypes.c_double), ("len_x", ctypes.c_double),
-                ("pert_mode", ctypes.c_int32), ("counter_streaming", ctypes.c_int32)]
+                ("pert_mode", ctypes.c_int32), ("counter_streaming", ctypes.c_int32),
+                ("profile", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("len_y", ctypes.c_double), ("ring_radius", ctypes.c_double),
+                ("ring_width", ctypes.c_double), ("column_charge", ctypes.c_double),
+                ("guide_bz", ctypes.c_double)]
 
 
 SIGNATURES = {
@@ -91,6 +95,7 @@ SIGNATURES = {
     "cpic_last_push_movers": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "cpic_device_errors": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
     "cpic_init_plasma": (ctypes.c_int, [ctypes.POINTER(PlasmaSpec), _I32, _I64] + [_P] * 8),
+    "cpic_init_column_fields": (ctypes.c_int, [ctypes.POINTER(PlasmaSpec), _I32, _P]),
     "cpic_tangent_create": (ctypes.c_int, [ctypes.POINTER(Config), _PP]),
     "cpic_tangent_destroy": (ctypes.c_int, [_P]),
     "cpic_tangent_contexts": (ctypes.c_int, [_P, _PP, _PP]),

@@ -23,6 +23,19 @@ inline uint64_t splitmix64(uint64_t x) {
 inline double unit_open_right(uint64_t r) { return (double)(r >> 11) * 0x1.0p-53; }  // [0,1)
 inline double unit_open_left(uint64_t r) { return (double)((r >> 11) + 1) * 0x1.0p-53; }  // (0,1]
 
+// Radial field of an infinite column of charge density
+// rho(r) = charge exp(-((r - r0) / sigma)^2) (Gauss's law with eps0 = 1):
+// E_r(r) = charge / r * int_0^r exp(-((s - r0) / sigma)^2) s ds, in closed
+// form with s = r0 + sigma t: sigma r0 sqrt(pi)/2 [erf t] - sigma^2/2 [e^{-t^2}].
+double column_radial_field(double r, double r0, double sigma, double charge) {
+  if (!(r > 0.0)) return 0.0;
+  const double t0 = -r0 / sigma, t1 = (r - r0) / sigma;
+  const double half_sqrt_pi = 0.88622692545275801364908374167057;
+  const double integral = sigma * r0 * half_sqrt_pi * (std::erf(t1) - std::erf(t0)) -
+                          0.5 * sigma * sigma * (std::exp(-t1 * t1) - std::exp(-t0 * t0));
+  return charge * integral / r;
+}
+
 template <class R>
 void fill(const cpic_plasma_spec& s, int64_t lo, int64_t hi, R* x, R* y, R* z, R* ux, R* uy,
           R* uz, R* w, int32_t* cell) {
@@ -44,18 +57,34 @@ void fill(const cpic_plasma_spec& s, int64_t lo, int64_t hi, R* x, R* y, R* z, R
     const int64_t c = p / s.nppc;
     const int64_t ix = 1 + c % s.nx, iy = 1 + (c / s.nx) % s.ny, iz = 1 + c / (s.nx * s.ny);
     const double sign = (s.counter_streaming && (p & 1)) ? -1.0 : 1.0;
-    double mod = 1.0;
-    if (s.pert_amp != 0.0 && s.len_x > 0.0) {
+    double mod = 1.0, weight = s.weight, drift[3] = {s.drift[0], s.drift[1], s.drift[2]};
+    if (s.profile == 1) {
+      // electron column: weight from the ring profile, E x B drift
+      const double dyc = s.len_y / (double)s.ny;
+      const double X = ((double)(ix - 1) + 0.5 * (r[0] + 1.0)) * dxc - 0.5 * s.len_x;
+      const double Y = ((double)(iy - 1) + 0.5 * (r[1] + 1.0)) * dyc - 0.5 * s.len_y;
+      const double rad = std::sqrt(X * X + Y * Y);
+      const double t = (rad - s.ring_radius) / s.ring_width;
+      weight *= std::exp(-t * t) *
+                (1.0 + s.pert_amp * std::cos((double)s.pert_mode * std::atan2(Y, X)));
+      const double er = column_radial_field(rad, s.ring_radius, s.ring_width, s.column_charge);
+      const double ex = rad > 0.0 ? er * X / rad : 0.0, ey = rad > 0.0 ? er * Y / rad : 0.0;
+      // v = E x B / B^2 with B = (0, 0, bz)
+      const double vx = ey / s.guide_bz, vy = -ex / s.guide_bz;
+      const double gam = 1.0 / std::sqrt(1.0 - (vx * vx + vy * vy));
+      drift[0] += gam * vx;
+      drift[1] += gam * vy;
+    } else if (s.pert_amp != 0.0 && s.len_x > 0.0) {
       const double xphys = ((double)(ix - 1) + 0.5 * (r[0] + 1.0)) * dxc;
       mod = 1.0 + s.pert_amp * std::sin(two_pi * (double)s.pert_mode * xphys / s.len_x);
     }
     x[p] = (R)r[0];
     y[p] = (R)r[1];
     z[p] = (R)r[2];
-    ux[p] = (R)(sign * s.drift[0] * mod + s.uth[0] * n[0]);
-    uy[p] = (R)(sign * s.drift[1] * mod + s.uth[1] * n[1]);
-    uz[p] = (R)(sign * s.drift[2] * mod + s.uth[2] * n[2]);
-    w[p] = (R)s.weight;
+    ux[p] = (R)(sign * drift[0] * mod + s.uth[0] * n[0]);
+    uy[p] = (R)(sign * drift[1] * mod + s.uth[1] * n[1]);
+    uz[p] = (R)(sign * drift[2] * mod + s.uth[2] * n[2]);
+    w[p] = (R)weight;
     cell[p] = (int32_t)(ix + X * (iy + Y * iz));
   }
 }
@@ -86,9 +115,57 @@ extern "C" int cpic_init_plasma(const cpic_plasma_spec* s, int32_t real_size, in
   if (real_size != 4 && real_size != 8) return CPIC_ERR_INVALID;
   if (s->nx < 1 || s->ny < 1 || s->nz < 1 || s->nppc < 1) return CPIC_ERR_INVALID;
   if (n != s->nx * s->ny * s->nz * s->nppc) return CPIC_ERR_INVALID;
+  if (s->profile != 0 && s->profile != 1) return CPIC_ERR_INVALID;
+  if (s->profile == 1 && !(s->len_x > 0.0 && s->len_y > 0.0 && s->ring_width > 0.0 &&
+                           s->guide_bz != 0.0))
+    return CPIC_ERR_INVALID;
   if (real_size == 4)
     fill_parallel<float>(*s, n, x, y, z, ux, uy, uz, w, cell);
   else
     fill_parallel<double>(*s, n, x, y, z, ux, uy, uz, w, cell);
+  return CPIC_OK;
+}
+
+namespace {
+
+template <class R>
+void column_fields(const cpic_plasma_spec& s, R* f) {
+  const int64_t X = s.nx + 2, Y = s.ny + 2, Z = s.nz + 2, N = X * Y * Z;
+  const double dx = s.len_x / (double)s.nx, dy = s.len_y / (double)s.ny;
+  for (int64_t iz = 0; iz < Z; ++iz)
+    for (int64_t iy = 0; iy < Y; ++iy)
+      for (int64_t ix = 0; ix < X; ++ix) {
+        const int64_t i = ix + X * (iy + Y * iz);
+        // Yee positions relative to the box centre: ex on the x edge at the
+        // voxel's (y-, z-) corner, ey on the y edge at its (x-, z-) corner
+        const double xc = ((double)ix - 0.5) * dx - 0.5 * s.len_x;
+        const double xl = ((double)ix - 1.0) * dx - 0.5 * s.len_x;
+        const double yc = ((double)iy - 0.5) * dy - 0.5 * s.len_y;
+        const double yl = ((double)iy - 1.0) * dy - 0.5 * s.len_y;
+        double r = std::sqrt(xc * xc + yl * yl);
+        double er = column_radial_field(r, s.ring_radius, s.ring_width, s.column_charge);
+        const double ex = r > 0.0 ? er * xc / r : 0.0;
+        r = std::sqrt(xl * xl + yc * yc);
+        er = column_radial_field(r, s.ring_radius, s.ring_width, s.column_charge);
+        const double ey = r > 0.0 ? er * yc / r : 0.0;
+        for (int k = 0; k < 9; ++k) f[k * N + i] = (R)0;
+        f[0 * N + i] = (R)ex;
+        f[1 * N + i] = (R)ey;
+        f[5 * N + i] = (R)s.guide_bz;
+      }
+}
+
+}  // namespace
+
+extern "C" int cpic_init_column_fields(const cpic_plasma_spec* s, int32_t real_size,
+                                       void* fields) {
+  if (!s || !fields) return CPIC_ERR_INVALID;
+  if (real_size != 4 && real_size != 8) return CPIC_ERR_INVALID;
+  if (s->nx < 1 || s->ny < 1 || s->nz < 1 || s->profile != 1) return CPIC_ERR_INVALID;
+  if (!(s->len_x > 0.0 && s->len_y > 0.0 && s->ring_width > 0.0)) return CPIC_ERR_INVALID;
+  if (real_size == 4)
+    column_fields<float>(*s, (float*)fields);
+  else
+    column_fields<double>(*s, (double*)fields);
   return CPIC_OK;
 }

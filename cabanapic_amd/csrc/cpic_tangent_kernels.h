@@ -186,7 +186,11 @@ __device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a, in
     pos[1] = ny;
     pos[2] = nz;
   } else {
-    atomicAdd(a.movers, 1u);
+    // one counter update per wave: the lowest active lane adds the number
+    // of lanes that leave their cell
+    const unsigned long long leaving = __ballot(1);
+    if ((threadIdx.x & 63) == (unsigned)__ffsll((long long)leaving) - 1)
+      atomicAdd(a.movers, (unsigned)__popcll(leaving));
     pos[0] = dx;
     pos[1] = dy;
     pos[2] = dz;

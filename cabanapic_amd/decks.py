@@ -123,11 +123,38 @@ class Deck:
         spec.len_x = float(self.len_x)
         spec.pert_mode = p.get("pert_mode", 0)
         spec.counter_streaming = 1 if p.get("counter_streaming") else 0
+        self._column_spec(spec)
         ptr = lambda a: a.ctypes.data_as(ctypes.c_void_p)
         _capi.check(_capi.load().cpic_init_plasma(
             ctypes.byref(spec), np.dtype(self.R).itemsize, n,
             *[ptr(out[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")], ptr(out["cell"])))
         return out
+
+    def _column_spec(self, spec):
+        p = self.plasma
+        if p.get("profile", 0) == 1:
+            spec.profile = 1
+            spec.len_y = float(self.len_y)
+            spec.ring_radius = p["ring_radius"]
+            spec.ring_width = p["ring_width"]
+            spec.column_charge = p["column_charge"]
+            spec.guide_bz = p["guide_bz"]
+
+    def load_fields(self):
+        """Initial fields ([9][Ng], reference member order) of a deck whose
+        plasma is an electron column: its radial E on the Yee edges of every
+        voxel and the guide field in cbz.  None for decks that start from
+        zero fields."""
+        if self.plasma is None or self.plasma.get("profile", 0) != 1:
+            return None
+        spec = _capi.PlasmaSpec()
+        spec.nx, spec.ny, spec.nz = self.nx, self.ny, self.nz
+        spec.len_x = float(self.len_x)
+        self._column_spec(spec)
+        f = np.empty((9, self.num_cells), self.R)
+        _capi.check(_capi.load().cpic_init_column_fields(
+            ctypes.byref(spec), np.dtype(self.R).itemsize, f.ctypes.data_as(ctypes.c_void_p)))
+        return f
 
 
 def _cube(deck, n, nppc, dx, steps):
@@ -200,8 +227,39 @@ def two_stream_1d(dtype=np.float64, nx=32, nppc_per_beam=32, steps=1000, seed=12
     return d.derive_params()
 
 
+def dioctron_3d(dtype=np.float32, nx=192, ny=192, nz=64, nppc=32, steps=1000, seed=3, bz=1.0,
+                amp=1e-3, mode=3, dx=0.1):
+    """Magnetized electron column (a 3D version of the reference's
+    decks/dioctron_3d.cxx): nx x ny x nz periodic cells of edge dx, nppc
+    electrons in every cell whose weights carry the ring density
+    n(r) = n0 exp(-((r - 0.3 Lx) / 0.05 Lx)^2) (1 + amp cos(mode theta)) about
+    the z axis through the box centre, n0 = 0.25 so that with the guide field
+    bz = 1 the peak w_ce / w_pe = 2; each electron drifts at the E x B
+    velocity of the column's own radial field, which also initializes ex, ey
+    in every voxel (ghosts included, as the reference's deck does); immobile
+    neutralizing ions are not part of the field solve; dt = 0.99 Courant."""
+    d = Deck(dtype)
+    R = d.R
+    d.nx, d.ny, d.nz = nx, ny, nz
+    d.nppc = nppc
+    d.num_steps = steps
+    d.len_x_global = R(F64(nx) * F64(dx))
+    d.len_y_global = R(F64(ny) * F64(dx))
+    d.len_z_global = R(F64(nz) * F64(dx))
+    d.dt = R(F64(0.99) * F64(d.courant_length(d.len_x_global, d.len_y_global, d.len_z_global,
+                                                nx, ny, nz)) / F64(d.c))
+    n0 = 0.25
+    d.n0 = R(n0)
+    L = float(d.len_x_global)
+    d.name = f"dioctron_{nx}x{ny}x{nz}x{nppc}"
+    d.plasma = dict(seed=seed, profile=1, ring_radius=0.3 * L, ring_width=0.05 * L,
+                    column_charge=float(d.qsp) * n0, guide_bz=bz, pert_amp=amp, pert_mode=mode)
+    return d.derive_params()
+
+
 CONFIGS = {
     "twostream1d": two_stream_1d,
     "weibel128": weibel_3d,
     "twostream64": two_stream_3d,
+    "dioctron": dioctron_3d,
 }

@@ -284,10 +284,32 @@ typedef struct cpic_plasma_spec {
   double len_x;
   int32_t pert_mode;
   int32_t counter_streaming;
+  /* profile 0: uniform density, drift modulated by
+   *   1 + pert_amp sin(2 pi pert_mode x / len_x).
+   * profile 1: an electron column along z about the box centre
+   *   (len_x/2, len_y/2): weight = weight * exp(-((r - ring_radius) /
+   *   ring_width)^2) * (1 + pert_amp cos(pert_mode theta)), and drift
+   *   velocity E x B / B^2 in the column's own radial field
+   *   E_r(r) = column_charge / r * int_0^r exp(-((s - ring_radius) /
+   *   ring_width)^2) s ds (column_charge = peak charge density) with a
+   *   uniform guide field guide_bz; u = gamma v plus the thermal spread. */
+  int32_t profile;
+  int32_t reserved;
+  double len_y;
+  double ring_radius, ring_width;
+  double column_charge;
+  double guide_bz;
 } cpic_plasma_spec;
 int cpic_init_plasma(const cpic_plasma_spec *spec, int32_t real_size,
                      int64_t n, void *x, void *y, void *z, void *ux, void *uy,
                      void *uz, void *w, int32_t *cell);
+/* Initial fields of a profile-1 spec (electron column): ex, ey of the
+ * column's radial field at their Yee edges in every voxel, ghosts included
+ * (as the reference's dioctron deck initializes every voxel,
+ * decks/dioctron_3d.cxx:40-64), cbz = guide_bz, all else 0.  fields is
+ * [9][Ng] of real_size bytes each, in the reference member order. */
+int cpic_init_column_fields(const cpic_plasma_spec *spec, int32_t real_size,
+                            void *fields);
 
 #ifdef __cplusplus
 }

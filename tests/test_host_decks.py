@@ -33,7 +33,8 @@ def test_regression_deck_initial_state_matches_oracle(real, tmp_path):
 @pytest.mark.parametrize("real", ["float", "double"])
 @pytest.mark.parametrize("deck,factory", [("weibel_3d", decks.weibel_3d),
                                           ("two_stream_3d", decks.two_stream_3d),
-                                          ("two_stream_1d", decks.two_stream_1d)])
+                                          ("two_stream_1d", decks.two_stream_1d),
+                                          ("dioctron_3d", decks.dioctron_3d)])
 def test_python_and_cpp_decks_agree(real, deck, factory, tmp_path):
     dtype = np.float32 if real == "float" else np.float64
     k, _ = host_driver.dump(deck, real, str(tmp_path / "c.bin"), particles=False)
@@ -53,6 +54,66 @@ def test_plasma_loader_matches_python_deck_particles(tmp_path):
     # counter-streaming: alternate drift signs, mean |ux| ~ drift
     assert np.mean(p["ux"][0::2]) > 0.15 and np.mean(p["ux"][1::2]) < -0.15
     assert np.all(np.abs(p["x"]) <= 1) and np.all(np.diff(p["cell"]) >= 0)
+
+
+def test_column_deck_fields_and_drifts():
+    """The electron-column loader (a reduced 48 x 48 x 4 grid of the
+    dioctron configuration): its radial field is the closed form of
+    Gauss's law for the ring, checked against numerical quadrature of
+    (1/r) int_0^r rho(s) s ds at the ex and ey Yee edges; particle weights
+    follow the ring profile with the cos(3 theta) perturbation; every
+    particle drifts at E x B / B^2 of the field at its own position."""
+    from scipy.integrate import quad
+    d = decks.dioctron_3d(np.float64, nx=48, ny=48, nz=4, nppc=4)
+    pl = d.plasma
+    r0, sig, rho0, bz = pl["ring_radius"], pl["ring_width"], pl["column_charge"], pl["guide_bz"]
+    L, dx = float(d.len_x), float(d.dx)
+
+    def er(r):
+        return rho0 / r * quad(lambda s: np.exp(-((s - r0) / sig) ** 2) * s, 0.0, r,
+                               epsabs=0, epsrel=1e-13, limit=200)[0]
+
+    f = d.load_fields()
+    X, Y = d.nx + 2, d.ny + 2
+    N = d.num_cells
+    assert f.shape == (9, N)
+    # the closed form cancels near the axis, where the field vanishes: the
+    # bound is relative to the peak field
+    tol = 1e-12 * np.abs(f[:2]).max()
+    rng = np.random.default_rng(5)
+    for v in rng.integers(0, X * Y, 40):
+        ix, iy = v % X, v // X
+        xc, xl = (ix - 0.5) * dx - L / 2, (ix - 1.0) * dx - L / 2
+        yc, yl = (iy - 0.5) * dx - L / 2, (iy - 1.0) * dx - L / 2
+        r = np.hypot(xc, yl)
+        assert abs(f[0, v] - er(r) * xc / r) <= tol, (ix, iy)
+        r = np.hypot(xl, yc)
+        assert abs(f[1, v] - er(r) * yc / r) <= tol, (ix, iy)
+    # every voxel (ghosts and all z layers) holds the same x-y field; B = (0, 0, bz)
+    assert np.array_equal(f[0, :X * Y], f[0, 3 * X * Y:4 * X * Y])
+    assert np.all(f[5] == bz) and not np.any(f[[2, 3, 4, 6, 7, 8]])
+    p = d.load_particles()
+    ix = p["cell"] % X
+    iy = (p["cell"] // X) % Y
+    xp = (ix - 1 + 0.5 * (p["x"] + 1)) * dx - L / 2
+    yp = (iy - 1 + 0.5 * (p["y"] + 1)) * dx - L / 2
+    r = np.hypot(xp, yp)
+    th = np.arctan2(yp, xp)
+    we = float(d.driver_constants()["we"])
+    prof = we * np.exp(-((r - r0) / sig) ** 2) * (1 + pl["pert_amp"] * np.cos(3 * th))
+    np.testing.assert_allclose(p["w"], prof, rtol=1e-12, atol=1e-300)
+    sel = rng.choice(r.size, 60, replace=False)
+    for i in sel:
+        e = er(r[i])
+        vx, vy = e * yp[i] / r[i] / bz, -e * xp[i] / r[i] / bz
+        g = 1 / np.sqrt(1 - vx * vx - vy * vy)
+        assert abs(p["ux"][i] - g * vx) <= tol / bz
+        assert abs(p["uy"][i] - g * vy) <= tol / bz
+    assert not np.any(p["uz"])
+    # the drift is azimuthal, counter-clockwise about +z for an electron column
+    assert np.all(np.abs(p["ux"] * xp + p["uy"] * yp) <= 1e-12 * (np.abs(p["ux"]) + np.abs(p["uy"])) * r)
+    assert np.all(xp * p["uy"] - yp * p["ux"] >= -tol / bz * r)
+    assert np.mean(xp * p["uy"] - yp * p["ux"]) > 0
 
 
 # The reference's own deck sources (src/input/deck.h interface), compiled
