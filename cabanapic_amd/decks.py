@@ -139,6 +139,8 @@ class Deck:
             spec.ring_width = p["ring_width"]
             spec.column_charge = p["column_charge"]
             spec.guide_bz = p["guide_bz"]
+            spec.pert_amp = p.get("pert_amp", 0.0)
+            spec.pert_mode = p.get("pert_mode", 0)
 
     def load_fields(self):
         """Initial fields ([9][Ng], reference member order) of a deck whose

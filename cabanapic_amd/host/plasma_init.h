@@ -60,6 +60,8 @@ class Plasma_Initializer : public Particle_Initializer {
     s.ring_width = ring_width;
     s.column_charge = column_charge;
     s.guide_bz = guide_bz;
+    s.pert_amp = pert_amp;
+    s.pert_mode = pert_mode;
   }
 };
 

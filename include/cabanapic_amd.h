@@ -287,12 +287,15 @@ typedef struct cpic_plasma_spec {
   /* profile 0: uniform density, drift modulated by
    *   1 + pert_amp sin(2 pi pert_mode x / len_x).
    * profile 1: an electron column along z about the box centre
-   *   (len_x/2, len_y/2): weight = weight * exp(-((r - ring_radius) /
-   *   ring_width)^2) * (1 + pert_amp cos(pert_mode theta)), and drift
-   *   velocity E x B / B^2 in the column's own radial field
-   *   E_r(r) = column_charge / r * int_0^r exp(-((s - ring_radius) /
-   *   ring_width)^2) s ds (column_charge = peak charge density) with a
-   *   uniform guide field guide_bz; u = gamma v plus the thermal spread. */
+   *   (len_x/2, len_y/2): weight = weight * g(X, Y) with the ring profile
+   *   g = exp(-((r - ring_radius) / ring_width)^2) (1 + pert_amp
+   *   cos(pert_mode theta)); the column's field is the periodic solution
+   *   of the discrete Poisson equation on the Yee grid for node charge
+   *   density column_charge * g minus its mean (the immobile ions
+   *   neutralize the box), the same in every z layer; each particle drifts
+   *   at E x B / B^2 of that field interpolated at its position as the push
+   *   interpolates it, with the guide field (0, 0, guide_bz); u = gamma v
+   *   plus the thermal spread. */
   int32_t profile;
   int32_t reserved;
   double len_y;
@@ -304,9 +307,9 @@ int cpic_init_plasma(const cpic_plasma_spec *spec, int32_t real_size,
                      int64_t n, void *x, void *y, void *z, void *ux, void *uy,
                      void *uz, void *w, int32_t *cell);
 /* Initial fields of a profile-1 spec (electron column): ex, ey of the
- * column's radial field at their Yee edges in every voxel, ghosts included
- * (as the reference's dioctron deck initializes every voxel,
- * decks/dioctron_3d.cxx:40-64), cbz = guide_bz, all else 0.  fields is
+ * column's periodic field (above) on their Yee edges in every voxel, ghosts
+ * included as periodic images (the reference's dioctron deck initializes
+ * every voxel, decks/dioctron_3d.cxx:40-64), cbz = guide_bz, all else 0.  fields is
  * [9][Ng] of real_size bytes each, in the reference member order. */
 int cpic_init_column_fields(const cpic_plasma_spec *spec, int32_t real_size,
                             void *fields);
