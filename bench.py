@@ -64,6 +64,9 @@ def run_gpu(deck, steps, warmup, compact_interval, history):
         sim.upload_fields(fields)
         del fields
     sp = deck.step_params(energy_interval=1, compact_interval=compact_interval)
+    # Gauss's law baseline (FP64 on the device), before any step
+    gargs = (float(deck.dx), float(deck.dy), float(deck.dz), float(deck.eps), float(deck.qsp))
+    sim.gauss_residual(*gargs, baseline=True)
     if warmup:
         sim.step(sp, warmup)
     # inside the timed region only the push stage is bracketed by events
@@ -80,6 +83,8 @@ def run_gpu(deck, steps, warmup, compact_interval, history):
     for name, v in sim.stage_times().items():
         if name != "push":
             stages[name] = v
+    change, rho = sim.gauss_residual(*gargs, baseline=False)
+    stages["gauss_residual_change_rel"] = change / rho
     errs = sim.device_errors()
     sim.close()
     return t1 - t0, hist, stages, errs
@@ -205,8 +210,10 @@ def main():
     push_gbs = bp / push_avg_s / 1e9
     step_gbs = (bp + bc) / (t32 / args.steps) / 1e9
 
+    gauss32 = st32.pop("gauss_residual_change_rel")
     extra = {"stage_ms_per_step": {k: round(v[1] / max(v[0], 1), 4) for k, v in st32.items()
                                    if v[0]},
+             "gauss_residual_change_rel": gauss32,
              "step_hbm_gbs": round(step_gbs, 1),
              "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
@@ -216,6 +223,7 @@ def main():
         t64, h64, st64, err64 = run_gpu(d64, args.fp64_steps, args.warmup, args.compact_interval,
                                         True)
         v64 = np_ * args.fp64_steps / t64
+        gauss64 = st64.pop("gauss_residual_change_rel")
         print(f"# fp64: {1e3 * t64 / args.fp64_steps:.3f} ms/step, {v64:.4g} particle-steps/s",
               file=sys.stderr, flush=True)
         # field energy of the FP32 run against the FP64 run of the same deck,
@@ -224,6 +232,7 @@ def main():
         e64 = h64[:nwin, 0] + h64[:nwin, 1]
         rel = np.abs(e32 - e64) / np.abs(e64)
         extra.update(fp64_value=v64, fp32_over_fp64=round(value / v64, 3),
+                     fp64_gauss_residual_change_rel=gauss64,
                      field_energy_rel_err_vs_fp64=float(rel.max()),
                      field_energy_window=f"steps {args.warmup + 1}-{args.warmup + nwin}")
 
@@ -245,6 +254,7 @@ def main():
         dc = decks.dioctron_3d(np.float32)
         tc, _, stc, errc = run_gpu(dc, args.column_steps, args.warmup, args.compact_interval,
                                    True)
+        extra.update(column_gauss_residual_change_rel=stc.pop("gauss_residual_change_rel"))
         if errc:
             raise SystemExit(f"device error flags {errc}")
         vc = dc.num_particles * args.column_steps / tc

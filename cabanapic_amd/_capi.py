@@ -94,6 +94,7 @@ SIGNATURES = {
     "cpic_stage_times": (ctypes.c_int, [_P, _P, _P]),
     "cpic_last_push_movers": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "cpic_device_errors": (ctypes.c_int, [_P, ctypes.POINTER(_I32)]),
+    "cpic_gauss_residual": (ctypes.c_int, [_P] + [_D] * 5 + [_I32, _P, _P]),
     "cpic_init_plasma": (ctypes.c_int, [ctypes.POINTER(PlasmaSpec), _I32, _I64] + [_P] * 8),
     "cpic_init_column_fields": (ctypes.c_int, [ctypes.POINTER(PlasmaSpec), _I32, _P]),
     "cpic_tangent_create": (ctypes.c_int, [ctypes.POINTER(Config), _PP]),

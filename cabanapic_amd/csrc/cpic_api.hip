@@ -100,6 +100,10 @@ struct cpic_context {
   double* hist_dev = nullptr;  // per-step energy rows of cpic_step
   long hist_cap = 0;           // rows allocated in hist_dev
   int n_partials = 0;
+  double* gauss_q = nullptr;     // node charge, then the residual (cpic_gauss_residual)
+  double* gauss_base = nullptr;  // baseline residual
+  unsigned long long* gauss_out = nullptr;  // [max |r| or |r - base|, max |rho / eps0|]
+  bool gauss_have_base = false;
 
   hipStream_t stream = nullptr;
   int device = 0;
@@ -600,7 +604,8 @@ int cpic_destroy(cpic_context* c) {
   void* ptrs[] = {c->rec1,   c->rec2,     c->rc1,      c->rc2,     c->ovf_rec,
                   c->ovf_cell, c->ovf_count, c->max_count, c->fields, c->interp,
                   c->acc,    c->partials, c->red_out,  c->err,     c->movers,
-                  c->hist_dev, c->dep, c->dep_words};
+                  c->hist_dev, c->dep, c->dep_words, c->gauss_q, c->gauss_base,
+                  c->gauss_out};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -905,6 +910,55 @@ int cpic_kinetic_energy(cpic_context* c, double me, double cc, double* ke) {
   HIP_TRY(hipMemcpyAsync(&h, c->red_out + 2, sizeof h, hipMemcpyDeviceToHost, c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
   if (ke) *ke = h;
+  return collect_timing(c);
+}
+
+int cpic_gauss_residual(cpic_context* c, double dx, double dy, double dz, double eps0,
+                        double qsp, int32_t mode, double* out, double* residual) {
+  if (int s = check_ctx(c)) return s;
+  if (mode != CPIC_GAUSS_BASELINE && mode != CPIC_GAUSS_CHANGE)
+    return fail(CPIC_ERR_INVALID, "Gauss check mode %d", (int)mode);
+  if (!(dx > 0 && dy > 0 && dz > 0 && eps0 > 0))
+    return fail(CPIC_ERR_INVALID, "cell sizes and eps0 must be positive");
+  if (mode == CPIC_GAUSS_CHANGE && !c->gauss_have_base)
+    return fail(CPIC_ERR_INVALID, "no Gauss baseline: call with CPIC_GAUSS_BASELINE first");
+  const size_t nc = (size_t)c->nc;
+  if (!c->gauss_q) {
+    if (int s = alloc((void**)&c->gauss_q, sizeof(double) * nc)) return s;
+    if (int s = alloc((void**)&c->gauss_base, sizeof(double) * nc)) return s;
+    if (int s = alloc((void**)&c->gauss_out, sizeof(unsigned long long) * 2)) return s;
+  }
+  int s = with_real(c, [&](auto r) {
+    using R = decltype(r);
+    if (int t = fill_zero<double>(c, c->gauss_q, nc)) return t;
+    if (int t = fill_zero<unsigned long long>(c, c->gauss_out, 2)) return t;
+    const int blocks = std::max(1, std::min(cdiv((long)nc, kBlock), 8192));
+    if (c->D > 0 && c->np > 0) {
+      const ColState<R> cs = col_state<R>(c, c->cur);
+      k_node_charge_columns<R><<<blocks, kBlock, 0, c->stream>>>(
+          cs, (int)c->nx, (int)c->ny, (int)c->nz, qsp, c->gauss_q);
+      CHECK_LAUNCH();
+    }
+    // the residual overwrites the node charge in place (same index)
+    k_gauss_residual<R><<<blocks, kBlock, 0, c->stream>>>(
+        (const R*)c->fields, c->grid, c->gauss_q, 1.0 / dx, 1.0 / dy, 1.0 / dz,
+        1.0 / (dx * dy * dz * eps0), c->gauss_q, c->gauss_base,
+        mode == CPIC_GAUSS_BASELINE ? 0 : 1, c->gauss_out);
+    CHECK_LAUNCH();
+    return CPIC_OK;
+  });
+  if (s) return s;
+  unsigned long long h[2];
+  HIP_TRY(hipMemcpyAsync(h, c->gauss_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  if (residual)
+    HIP_TRY(hipMemcpyAsync(residual, c->gauss_q, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  if (mode == CPIC_GAUSS_BASELINE) c->gauss_have_base = true;
+  if (out) {
+    std::memcpy(&out[0], &h[0], sizeof(double));
+    std::memcpy(&out[1], &h[1], sizeof(double));
+  }
   return collect_timing(c);
 }
 

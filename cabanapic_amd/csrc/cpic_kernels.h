@@ -437,4 +437,56 @@ k_reduce_final(const double* __restrict__ partials, int nblocks, double s0, doub
   }
 }
 
+// Largest of a non-negative double over the wave, then one 64-bit atomic
+// max per wave on its bit pattern (the order of non-negative doubles is
+// the order of their bit patterns as unsigned integers).
+__device__ __forceinline__ void wave_max_store(double v, unsigned long long* out) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(v));
+}
+
+// Gauss residual at the interior nodes in FP64:
+//   r(v) = sum_a (e_a(v) - e_a(v - a)) / d_a  -  q(v) / (dx dy dz eps0),
+// where the x edge of voxel v (ex, src/types.h:138-172) joins node v to node
+// v + x, and v - a in the low ghost layer is read at its periodic image
+// (the reference never writes the low ghost layer of E).  q: node charge,
+// interior order.  mode 0 stores r in base; mode 1 compares with base.
+// out[0] = max |r| (mode 0) or max |r - base| (mode 1), out[1] = max
+// |q| / (dx dy dz eps0); res (may be null) receives r.
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_gauss_residual(const R* __restrict__ fields, Grid g, const double* __restrict__ q,
+                 double inv_dx, double inv_dy, double inv_dz, double qscale,
+                 double* __restrict__ res, double* __restrict__ base, int mode,
+                 unsigned long long* __restrict__ out) {
+  const int nint = g.nx * g.ny * g.nz;
+  const size_t N = g.ncells;
+  double m0 = 0, m1 = 0;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nint; t += gridDim.x * blockDim.x) {
+    const int x = 1 + t % g.nx;
+    const int y = 1 + (t / g.nx) % g.ny;
+    const int z = 1 + t / (g.nx * g.ny);
+    const int i = voxel(g, x, y, z);
+    const int ixm = voxel(g, x == 1 ? g.nx : x - 1, y, z);
+    const int iym = voxel(g, x, y == 1 ? g.ny : y - 1, z);
+    const int izm = voxel(g, x, y, z == 1 ? g.nz : z - 1);
+    const double div = ((double)fields[i] - (double)fields[ixm]) * inv_dx +
+                       ((double)fields[N + i] - (double)fields[N + iym]) * inv_dy +
+                       ((double)fields[2 * N + i] - (double)fields[2 * N + izm]) * inv_dz;
+    const double rho = q[t] * qscale;
+    const double r = div - rho;
+    if (res) res[t] = r;
+    if (mode == 0) {
+      base[t] = r;
+      m0 = fmax(m0, fabs(r));
+    } else {
+      m0 = fmax(m0, fabs(r - base[t]));
+    }
+    m1 = fmax(m1, fabs(rho));
+  }
+  wave_max_store(m0, out);
+  wave_max_store(m1, out + 1);
+}
+
 }  // namespace cpic

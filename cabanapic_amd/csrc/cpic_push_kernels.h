@@ -1203,4 +1203,45 @@ k_kinetic_columns(ColState<R> cs, int nc, double* __restrict__ partials) {
   block_sum_store<1>(v, partials + blockIdx.x);
 }
 
+// Cloud-in-cell charge of the columns' particles at the nodes, FP64.  Node
+// v is the low corner of voxel v.  A particle at offset o in [-1, 1] of
+// cell i gives the fraction (1 - o)/2 of its charge w qsp to node i and
+// (1 + o)/2 to node i + 1 along each axis (the weights of the charge whose
+// flux the deposit of src/move_p.h:156-190 carries).  Nodes n+1 are the
+// periodic images of nodes 1.  One lane per cell: the 8 corner sums in FP64
+// registers, then one FP64 atomic per corner into q (interior node order,
+// x fastest, 0-based).
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_node_charge_columns(ColState<R> cs, int nx, int ny, int nz, double qsp,
+                      double* __restrict__ q) {
+  const int nc = nx * ny * nz;
+  for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nc; ci += gridDim.x * blockDim.x) {
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int n = cs.cnt_in[ci];
+    unsigned long long m = 0;
+    for (int j = 0; j < n; ++j) {
+      if ((j & 63) == 0) m = cs.mask_in[(size_t)(j >> 6) * cs.nc_pad + ci];
+      if (!((m >> (j & 63)) & 1ull)) continue;
+      const R* p = cs.in + col_slot(ci, j, cs.D) * kSlotLen;
+      const double ox = p[0], oy = p[1], oz = p[2];
+      const double qq = (double)p[6] * qsp;
+      const double wx[2] = {0.5 * (1.0 - ox), 0.5 * (1.0 + ox)};
+      const double wy[2] = {0.5 * (1.0 - oy), 0.5 * (1.0 + oy)};
+      const double wz[2] = {0.5 * (1.0 - oz), 0.5 * (1.0 + oz)};
+#pragma unroll
+      for (int k = 0; k < 8; ++k) s[k] += qq * wx[k & 1] * wy[(k >> 1) & 1] * wz[k >> 2];
+    }
+    const int ix = ci % nx, iy = (ci / nx) % ny, iz = ci / (nx * ny);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      if (s[k] == 0.0) continue;
+      const int jx = ix + (k & 1) == nx ? 0 : ix + (k & 1);
+      const int jy = iy + ((k >> 1) & 1) == ny ? 0 : iy + ((k >> 1) & 1);
+      const int jz = iz + (k >> 2) == nz ? 0 : iz + (k >> 2);
+      atomicAdd(q + jx + nx * (jy + ny * jz), s[k]);
+    }
+  }
+}
+
 }  // namespace cpic

@@ -35,6 +35,8 @@ import numpy as np
 from . import _capi
 from ._capi import BOUNDARY_PERIODIC, CpicError, StepParams, check
 
+GAUSS_BASELINE, GAUSS_CHANGE = 0, 1  # cpic_gauss_residual modes
+
 PARTICLE_MEMBERS = ("x", "y", "z", "ux", "uy", "uz", "w")
 FIELD_MEMBERS = ("ex", "ey", "ez", "cbx", "cby", "cbz", "jfx", "jfy", "jfz")
 INTERP_MEMBERS = ("ex", "dexdy", "dexdz", "d2exdydz", "ey", "deydz", "deydx", "d2eydzdx",
@@ -191,6 +193,20 @@ class Simulation:
         e, b = ctypes.c_double(), ctypes.c_double()
         check(self.lib.cpic_energies(self._h, ctypes.byref(e), ctypes.byref(b)))
         return e.value, b.value
+
+    def gauss_residual(self, dx, dy, dz, eps0, qsp, baseline, residual=False):
+        """Gauss's law check in FP64 (cpic_gauss_residual): with baseline
+        True stores the residual div E - rho / eps0 of every interior node
+        and returns (max |r|, max |rho / eps0|), else returns (max |r -
+        baseline|, max |rho / eps0|); with residual True also the residual
+        array, shape (nz, ny, nx)."""
+        out = np.zeros(2)
+        r = np.empty((self.nz, self.ny, self.nx)) if residual else None
+        check(self.lib.cpic_gauss_residual(
+            self._h, float(dx), float(dy), float(dz), float(eps0), float(qsp),
+            GAUSS_BASELINE if baseline else GAUSS_CHANGE, _ptr(out),
+            _ptr(r) if residual else None))
+        return (out[0], out[1], r) if residual else (out[0], out[1])
 
     def e_energy(self):
         return self.energies()[0]

@@ -169,6 +169,25 @@ int cpic_energies(cpic_context *ctx, double *e_energy, double *b_energy);
  * counterpart).  Synchronous. */
 int cpic_kinetic_energy(cpic_context *ctx, double me, double c, double *ke);
 
+/* Gauss's law check in FP64 on the device (the reference's path has none;
+ * its charge-conserving deposit, src/move_p.h:156-190, keeps the residual
+ * constant under the EM solver).  At every interior node v (the low corner
+ * of voxel v):
+ *   r(v) = sum_a (e_a(v) - e_a(v - a)) / d_a  -  rho(v) / eps0,
+ * with the x edge of voxel v joining nodes v and v + x, the periodic image
+ * read for v - a in the low ghost layer, and rho the cloud-in-cell charge
+ * density of the particles (weight w qsp, fraction (1 -+ o)/2 per axis to
+ * the low/high node, o the in-cell offset), divided by dx dy dz.
+ * CPIC_GAUSS_BASELINE stores r as the baseline, out[0] = max |r|;
+ * CPIC_GAUSS_CHANGE needs a baseline, out[0] = max |r - baseline|.  Both:
+ * out[1] = max |rho / eps0|.  residual (may be NULL) receives r for the
+ * nx*ny*nz nodes, x fastest, node (1,1,1) first.  Synchronous. */
+#define CPIC_GAUSS_BASELINE 0
+#define CPIC_GAUSS_CHANGE 1
+int cpic_gauss_residual(cpic_context *ctx, double dx, double dy, double dz,
+                        double eps0, double qsp, int32_t mode, double *out,
+                        double *residual);
+
 /* Sort of the particles by cell (the commented-out Cabana::sortByKey of
  * example/example.cpp:226-228).  Particles are binned by cell from upload
  * on and every push keeps them binned, so this only validates the context;
