@@ -38,7 +38,7 @@ _PP = ctypes.POINTER(ctypes.c_void_p)
 class Config(ctypes.Structure):
     _fields_ = [("real_size", ctypes.c_int32), ("num_ghosts", ctypes.c_int32),
                 ("nx", ctypes.c_int64), ("ny", ctypes.c_int64), ("nz", ctypes.c_int64),
-                ("max_particles", ctypes.c_int64)]
+                ("max_particles", ctypes.c_int64), ("column_rows", ctypes.c_int64)]
 
 
 class StepParams(ctypes.Structure):
@@ -104,6 +104,7 @@ SIGNATURES = {
     "cpic_tangent_download_particles": (ctypes.c_int, [_P, _I64] + [_P] * 9),
     "cpic_tangent_push": (ctypes.c_int, [_P, _D, _D, _D, _D, _D, _I32]),
     "cpic_tangent_step": (ctypes.c_int, [_P, ctypes.POINTER(StepParams), _I64, _P]),
+    "cpic_tangent_set_graph": (ctypes.c_int, [_P, _I32]),
     "cpic_tangent_last_push_movers": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
 }
 

@@ -62,6 +62,7 @@ struct cpic_context {
   long nc = 0;      // interior cells
   long nc_pad = 0;  // interior cells rounded up to whole blocks (kBlock lanes)
   long cap = 0;     // particle capacity declared at creation
+  long column_rows = 0;  // requested rows per column (0 = automatic)
   long np = 0;      // particles resident (conserved by the push)
   Grid grid{};
   CellGrid cg{};
@@ -528,6 +529,8 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
     return fail(CPIC_ERR_INVALID, "grid too large for 32-bit voxel indexing");
   if (cfg->max_particles < 0 || cfg->max_particles > 2147483647L)
     return fail(CPIC_ERR_INVALID, "max_particles out of range");
+  if (cfg->column_rows < 0 || cfg->column_rows > (1L << 20))
+    return fail(CPIC_ERR_INVALID, "column_rows out of range [0, 2^20]");
 
   cpic_context* c = new (std::nothrow) cpic_context();
   if (!c) return fail(CPIC_ERR_INVALID, "out of host memory");
@@ -537,6 +540,7 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
   c->nz = cfg->nz;
   c->ncells = X * Y * Z;
   c->cap = cfg->max_particles;
+  c->column_rows = (long)cfg->column_rows;
   c->grid.nx = (int)c->nx;
   c->grid.ny = (int)c->ny;
   c->grid.nz = (int)c->nz;
@@ -652,8 +656,12 @@ int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void*
     ci[p] = (int)((ix - 1) + c->nx * ((iy - 1) + c->ny * (iz - 1)));
     maxc = std::max(maxc, ++count[ci[p]]);
   }
-  auto up8 = [](int v) { return (v + 7) / 8 * 8; };
-  const int D = up8(std::max({maxc * 5 / 2, maxc + 48, 16}));
+  auto up8 = [](long v) { return (int)((v + 7) / 8 * 8); };
+  int D = up8(std::max({maxc * 5 / 2, maxc + 48, 16}));
+  // every particle in one cell, where that costs at most 64 MB of columns
+  const double worst_bytes = (double)c->nc_pad * (double)up8(n) * kSlotLen * (double)c->rsz();
+  if (worst_bytes <= 64.0 * 1024 * 1024) D = std::max(D, up8(n));
+  if (c->column_rows > 0) D = std::max(up8(c->column_rows), up8(maxc));
   const int M1 = std::min(D, std::max(16, up8(maxc / 2)));
   const int M2 = std::min(D, 8);
   if (int s = ensure_columns(c, D, M1, M2)) return s;

@@ -239,6 +239,18 @@ __global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
   deposit_runs(a, key, own);
 }
 
+// Appends the step's row (E, B, dE, dB) to the history at the row a
+// device counter names, then advances the counter: the history of graph
+// replays, whose kernel arguments cannot change from step to step.
+__global__ void k_append_row(const double* __restrict__ row, double* __restrict__ hist,
+                             int* __restrict__ counter, int cap) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int r = *counter;
+  if (r < cap)
+    for (int k = 0; k < 4; ++k) hist[4 * (size_t)r + k] = row[k];
+  *counter = r + 1;
+}
+
 // sum over interior cells of E . tE and B . tB -> partials[block][2] (FP64):
 // the derivatives of the field energies 0.5 sum E^2 and 0.5 sum B^2
 template <class R>

@@ -260,6 +260,11 @@ int main(int argc, char** argv) {
     CPIC_CALL(cpic_uncenter_particles(ctx, qdt_2mc));
   }
 
+  // Gauss's law in FP64 on the device: the charge-conserving deposit keeps
+  // div E - rho / eps0 constant at every node; its change is printed at the end
+  double gauss[2] = {0, 0};
+  CPIC_CALL(cpic_gauss_residual(ctx, dx, dy, dz, eps0, qsp, CPIC_GAUSS_BASELINE, gauss, nullptr));
+
   if (stages) {
     for (int step = 1; step <= num_steps; step++) {
       CPIC_CALL(cpic_load_interpolator_array(ctx));
@@ -309,6 +314,9 @@ int main(int argc, char** argv) {
   }
   if (fptr) std::fclose(fptr);
   if (fpfd) std::fclose(fpfd);
+  CPIC_CALL(cpic_gauss_residual(ctx, dx, dy, dz, eps0, qsp, CPIC_GAUSS_CHANGE, gauss, nullptr));
+  std::fprintf(stderr, "#Gauss residual: max |change| %e, max |rho/eps0| %e (relative %e)\n",
+               gauss[0], gauss[1], gauss[1] > 0 ? gauss[0] / gauss[1] : 0.0);
   int32_t flags = 0;
   CPIC_CALL(cpic_device_errors(ctx, &flags));
   CPIC_CALL(cpic_destroy(ctx));

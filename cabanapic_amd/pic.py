@@ -51,7 +51,8 @@ def _ptr(a):
 class Simulation:
     """One simulation's device state on the current HIP device."""
 
-    def __init__(self, nx, ny, nz, max_particles, dtype=np.float32, num_ghosts=1):
+    def __init__(self, nx, ny, nz, max_particles, dtype=np.float32, num_ghosts=1,
+                 column_rows=0):
         self.lib = _capi.load()
         self.dtype = np.dtype(dtype)
         if self.dtype not in (np.float32, np.float64):
@@ -59,7 +60,7 @@ class Simulation:
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.ncells = (self.nx + 2) * (self.ny + 2) * (self.nz + 2)
         cfg = _capi.Config(self.dtype.itemsize, num_ghosts, self.nx, self.ny, self.nz,
-                           int(max_particles))
+                           int(max_particles), int(column_rows))
         h = ctypes.c_void_p()
         check(self.lib.cpic_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -338,6 +339,10 @@ class TangentSimulation:
         n = ctypes.c_int64()
         check(self.lib.cpic_tangent_last_push_movers(self._h, ctypes.byref(n)))
         return n.value
+
+    def set_graph(self, enabled):
+        """Replay the step as a HIP graph (default) or launch every stage."""
+        check(self.lib.cpic_tangent_set_graph(self._h, 1 if enabled else 0))
 
 
 def make_step_params(qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, dx, dy, dz, dt, px, py, pz,

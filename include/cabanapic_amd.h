@@ -50,6 +50,14 @@ typedef struct cpic_config {
   int32_t num_ghosts;    /* must be 1 (deck.h:267)                         */
   int64_t nx, ny, nz;    /* interior cells per axis                        */
   int64_t max_particles; /* particle capacity of the context               */
+  /* Rows of each cell's particle column (no reference counterpart: the
+   * reference stores particles in one flat array).  0 = automatic: 2.5x
+   * the largest cell count at upload (at least +48), or every particle of
+   * the context where that costs at most 64 MB of columns.  A push that
+   * would put more particles into a cell than its column holds fails with
+   * CPIC_ERR_DEVICE; bunching physics (a nonlinear two-stream phase can
+   * reach 4x the mean count) may need a larger value. */
+  int64_t column_rows;
 } cpic_config;
 
 /* Constants of one step, as computed by the driver
@@ -279,6 +287,12 @@ int cpic_tangent_push(cpic_tangent *tg, double qdt_2mc, double cdt_dx,
 int cpic_tangent_step(cpic_tangent *tg, const cpic_step_params *sp,
                       int64_t nsteps, double *hist);
 int cpic_tangent_last_push_movers(cpic_tangent *tg, int64_t *movers);
+/* Graph replay of cpic_tangent_step (no reference counterpart): with
+ * enabled != 0 (the default) the step is captured once into a HIP graph for
+ * its parameters, particle count and history buffer, and replayed once per
+ * step, so that a small grid is not bound by kernel launches; 0 launches
+ * every stage as it goes.  Steps with stage timing on always launch. */
+int cpic_tangent_set_graph(cpic_tangent *tg, int32_t enabled);
 
 /* ---- deck support (host only, no device call) -------------------------- */
 

@@ -10,6 +10,7 @@ import pytest
 from cabanapic_amd import _capi
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cabanapic_amd.h")
 
 
 def declared_symbols():
@@ -41,10 +42,32 @@ def test_binding_covers_header():
     assert declared_symbols() == set(_capi.SIGNATURES)
 
 
-def test_struct_layouts_match_header():
-    # cpic_config: 2 int32 + 4 int64; cpic_step_params: 15 doubles + 4 int32
-    assert ctypes.sizeof(_capi.Config) == 8 + 4 * 8
-    assert ctypes.sizeof(_capi.StepParams) == 15 * 8 + 4 * 4
+def test_struct_layouts_match_header(tmp_path):
+    """Every struct of the header, compiled by gcc: its size and the offset
+    of every member equal those of the ctypes binding."""
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"cpic_config": _capi.Config, "cpic_step_params": _capi.StepParams,
+               "cpic_plasma_spec": _capi.PlasmaSpec}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", f'#include "{HEADER}"',
+             "int main(void) {"]
+    for name, cls in structs.items():
+        lines.append(f'  printf("{name} %zu\\n", sizeof({name}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{name}.{field} %zu\\n", offsetof({name}, {field}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-o", str(exe), str(src)], check=True)
+    out = dict(l.split() for l in subprocess.run([str(exe)], check=True, capture_output=True,
+                                                 text=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(out[name]) == ctypes.sizeof(cls), name
+        for field, _ in cls._fields_:
+            assert int(out[f"{name}.{field}"]) == getattr(cls, field).offset, (name, field)
 
 
 def test_last_error_without_device_call():
