@@ -63,6 +63,10 @@ struct cpic_context {
   long nc_pad = 0;  // interior cells rounded up to whole blocks (kBlock lanes)
   long cap = 0;     // particle capacity declared at creation
   long column_rows = 0;  // requested rows per column (0 = automatic)
+  long min_rows = 0;     // rows per column the next upload keeps at least (regrowth)
+  int* occ_host = nullptr;        // pinned copy of the last push's max_count
+  hipEvent_t occ_event = nullptr;  // recorded after that copy
+  bool occ_pending = false;        // a copy is in flight
   long np = 0;      // particles resident (conserved by the push)
   Grid grid{};
   CellGrid cg{};
@@ -507,6 +511,50 @@ int check_device_errors(cpic_context* c) {
   return CPIC_OK;
 }
 
+// ---- column headroom -------------------------------------------------------
+// Each push copies its largest input column occupancy (rows in use) to
+// pinned host memory.  Before the next push the host reads the copy; a
+// column within max(8, D/10) rows of its capacity makes the host re-lay the
+// columns out (download, then upload with at least 1.5x the rows), so that
+// particles bunching in one cell (a nonlinear two-stream phase reaches 4x
+// the mean count) do not overflow it.  A push that overflows anyway still
+// fails with kErrColumnFull.
+
+int note_occupancy(cpic_context* c) {
+  if (c->D == 0 || c->np == 0) return CPIC_OK;
+  if (!c->occ_host) {
+    HIP_TRY(hipHostMalloc((void**)&c->occ_host, sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&c->occ_event, hipEventDisableTiming));
+  }
+  HIP_TRY(hipMemcpyAsync(c->occ_host, c->max_count, sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipEventRecord(c->occ_event, c->stream));
+  c->occ_pending = true;
+  return CPIC_OK;
+}
+
+int regrow_columns(cpic_context* c) {
+  const int64_t n = c->np;
+  const size_t rs = c->rsz();
+  std::vector<unsigned char> m(7 * rs * (size_t)n);
+  std::vector<int32_t> cell((size_t)n);
+  void* d[7];
+  for (int k = 0; k < 7; ++k) d[k] = m.data() + (size_t)k * rs * (size_t)n;
+  if (int s = cpic_download_particles(c, n, d[0], d[1], d[2], d[3], d[4], d[5], d[6],
+                                      cell.data()))
+    return s;
+  c->min_rows = std::max<long>(c->min_rows, (long)c->D * 3 / 2);
+  return cpic_upload_particles(c, n, d[0], d[1], d[2], d[3], d[4], d[5], d[6], cell.data());
+}
+
+int check_headroom(cpic_context* c) {
+  if (!c->occ_pending) return CPIC_OK;
+  HIP_TRY(hipEventSynchronize(c->occ_event));
+  c->occ_pending = false;
+  if (c->D == 0 || *c->occ_host < c->D - std::max(8, c->D / 10)) return CPIC_OK;
+  return regrow_columns(c);
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -612,6 +660,8 @@ int cpic_destroy(cpic_context* c) {
                   c->gauss_out};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (c->occ_event) (void)hipEventDestroy(c->occ_event);
+  if (c->occ_host) (void)hipHostFree(c->occ_host);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return CPIC_OK;
@@ -661,7 +711,8 @@ int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void*
   // every particle in one cell, where that costs at most 64 MB of columns
   const double worst_bytes = (double)c->nc_pad * (double)up8(n) * kSlotLen * (double)c->rsz();
   if (worst_bytes <= 64.0 * 1024 * 1024) D = std::max(D, up8(n));
-  if (c->column_rows > 0) D = std::max(up8(c->column_rows), up8(maxc));
+  if (c->column_rows > 0) D = std::max(D, up8(c->column_rows));
+  if (c->min_rows > 0) D = std::max(D, up8(c->min_rows));
   const int M1 = std::min(D, std::max(16, up8(maxc / 2)));
   const int M2 = std::min(D, 8);
   if (int s = ensure_columns(c, D, M1, M2)) return s;
@@ -836,10 +887,13 @@ int cpic_push(cpic_context* c, double qdt_2mc, double cdt_dx, double cdt_dy, dou
   if (int s = check_periodic(boundary)) return s;
   return with_real(c, [&](auto r) {
     using R = decltype(r);
+    if (int t = check_headroom(c)) return t;
     // the first push after an upload and then every kCompactInterval-th one
     // pack the columns
     const int compact = (c->pushes++ % kCompactInterval == 0) ? 1 : 0;
-    return stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1, compact);
+    if (int t = stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1, compact))
+      return t;
+    return note_occupancy(c);
   });
 }
 
@@ -1006,11 +1060,13 @@ int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, doubl
       // the push overwrites every interior accumulator, so the clear is only
       // needed while ghost accumulators may hold data
       if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
+      if ((t = check_headroom(c))) return t;
       const int ci = sp->compact_interval > 0 ? sp->compact_interval : kCompactInterval;
       const int compact = (c->pushes++ % ci == 0) ? 1 : 0;
       if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
                              (R)sp->qsp, 0, compact)))
         return t;
+      if ((t = note_occupancy(c))) return t;
       if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;
       if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
       if ((t = stage_advance_e<R>(c, px, py, pz, (R)sp->dt_eps0))) return t;

@@ -50,13 +50,15 @@ typedef struct cpic_config {
   int32_t num_ghosts;    /* must be 1 (deck.h:267)                         */
   int64_t nx, ny, nz;    /* interior cells per axis                        */
   int64_t max_particles; /* particle capacity of the context               */
-  /* Rows of each cell's particle column (no reference counterpart: the
-   * reference stores particles in one flat array).  0 = automatic: 2.5x
-   * the largest cell count at upload (at least +48), or every particle of
-   * the context where that costs at most 64 MB of columns.  A push that
-   * would put more particles into a cell than its column holds fails with
-   * CPIC_ERR_DEVICE; bunching physics (a nonlinear two-stream phase can
-   * reach 4x the mean count) may need a larger value. */
+  /* Least rows of each cell's particle column (no reference counterpart:
+   * the reference stores particles in one flat array); 0 = automatic.  At
+   * upload a column holds 2.5x the largest cell count (at least +48 rows),
+   * or every particle of the context where that costs at most 64 MB of
+   * columns, and at least column_rows.  Before each push the library
+   * checks the previous push's fullest column; within max(8, rows/10) of
+   * the capacity it re-lays the columns out with 1.5x the rows (bunching
+   * physics: a nonlinear two-stream phase reaches 4x the mean count).  A
+   * push that still overflows a column fails with CPIC_ERR_DEVICE. */
   int64_t column_rows;
 } cpic_config;
 
