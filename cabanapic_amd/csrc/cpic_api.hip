@@ -707,7 +707,9 @@ int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void*
     maxc = std::max(maxc, ++count[ci[p]]);
   }
   auto up8 = [](long v) { return (int)((v + 7) / 8 * 8); };
-  int D = up8(std::max({maxc * 5 / 2, maxc + 48, 16}));
+  // rounded up to whole 64-row mask words: the masks and departure words are
+  // allocated per word, so the rows up to the next word cost only slots
+  int D = (std::max({maxc * 5 / 2, maxc + 48, 16}) + 63) / 64 * 64;
   // every particle in one cell, where that costs at most 64 MB of columns
   const double worst_bytes = (double)c->nc_pad * (double)up8(n) * kSlotLen * (double)c->rsz();
   if (worst_bytes <= 64.0 * 1024 * 1024) D = std::max(D, up8(n));

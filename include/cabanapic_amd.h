@@ -52,8 +52,8 @@ typedef struct cpic_config {
   int64_t max_particles; /* particle capacity of the context               */
   /* Least rows of each cell's particle column (no reference counterpart:
    * the reference stores particles in one flat array); 0 = automatic.  At
-   * upload a column holds 2.5x the largest cell count (at least +48 rows),
-   * or every particle of the context where that costs at most 64 MB of
+   * upload a column holds 2.5x the largest cell count (at least +48 rows,
+   * rounded up to a multiple of 64), or every particle of the context where that costs at most 64 MB of
    * columns, and at least column_rows.  Before each push the library
    * checks the previous push's fullest column; within max(8, rows/10) of
    * the capacity it re-lays the columns out with 1.5x the rows (bunching
