@@ -514,7 +514,7 @@ int check_device_errors(cpic_context* c) {
 // ---- column headroom -------------------------------------------------------
 // Each push copies its largest input column occupancy (rows in use) to
 // pinned host memory.  Before the next push the host reads the copy; a
-// column within max(8, D/10) rows of its capacity makes the host re-lay the
+// column within max(8, D/4) rows of its capacity makes the host re-lay the
 // columns out (download, then upload with at least 1.5x the rows), so that
 // particles bunching in one cell (a nonlinear two-stream phase reaches 4x
 // the mean count) do not overflow it.  A push that overflows anyway still
@@ -551,7 +551,7 @@ int check_headroom(cpic_context* c) {
   if (!c->occ_pending) return CPIC_OK;
   HIP_TRY(hipEventSynchronize(c->occ_event));
   c->occ_pending = false;
-  if (c->D == 0 || *c->occ_host < c->D - std::max(8, c->D / 10)) return CPIC_OK;
+  if (c->D == 0 || *c->occ_host < c->D - std::max(8, c->D / 4)) return CPIC_OK;
   return regrow_columns(c);
 }
 

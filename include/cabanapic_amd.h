@@ -55,7 +55,7 @@ typedef struct cpic_config {
    * upload a column holds 2.5x the largest cell count (at least +48 rows,
    * rounded up to a multiple of 64), or every particle of the context where that costs at most 64 MB of
    * columns, and at least column_rows.  Before each push the library
-   * checks the previous push's fullest column; within max(8, rows/10) of
+   * checks the previous push's fullest column; within max(8, rows/4) of
    * the capacity it re-lays the columns out with 1.5x the rows (bunching
    * physics: a nonlinear two-stream phase reaches 4x the mean count).  A
    * push that still overflows a column fails with CPIC_ERR_DEVICE. */

@@ -1,0 +1,67 @@
+"""Column capacity under bunching (cpic_config.column_rows, automatic).
+
+Weightless particles (no charge, so the fields stay exactly zero and every
+push is free streaming) converge on one plane: the particle at distance d
+from the focus moves d / T per step towards it, so the count of the focus
+cells grows as 1 / (1 - t/T).  The columns sized at upload (64 rows for 8
+per cell) must grow on the way; after the run every particle must equal the
+oracle's free-streaming run bit for bit, with no particle lost.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib
+import states
+from cabanapic_amd import Simulation
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ, PPC = 64, 4, 4, 8
+T = 128  # steps to the focus
+STEPS = 119  # 0.93 T: the focus cells hold about 14x their initial count
+
+
+def focusing_state(dtype):
+    rng = np.random.default_rng(17)
+    nc = NX * NY * NZ
+    n = nc * PPC
+    st = oracle_lib.OracleState(NX, NY, NZ, n, dtype)
+    st.fields[:] = 0
+    c = np.repeat(np.arange(nc), PPC)
+    ix, iy, iz = 1 + c % NX, 1 + (c // NX) % NY, 1 + c // (NX * NY)
+    st.cell[:] = ix + (NX + 2) * (iy + (NY + 2) * iz)
+    for m in ("x", "y", "z"):
+        st.p[m][:] = rng.uniform(-1, 1, n).astype(dtype)
+    # cell widths from the focus plane x = NX/2 (in cells), velocity d / (T cdt)
+    xc = (ix - 1) + 0.5 * (st.p["x"].astype(np.float64) + 1.0)
+    d = NX / 2 - xc
+    cdt = 0.5
+    v = d / (T * cdt)
+    st.p["ux"][:] = (v / np.sqrt(1.0 - v * v)).astype(dtype)
+    st.p["uy"][:] = 0
+    st.p["uz"][:] = 0
+    st.p["w"][:] = 0
+    k = states.step_consts(dtype, cdt=(cdt, cdt, cdt), dt=0.5)
+    return st, k
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64], ids=["f32", "f64"])
+def test_columns_grow_under_bunching(dtype):
+    st, k = focusing_state(dtype)
+    counts0 = np.bincount(st.cell).max()
+    with Simulation(NX, NY, NZ, st.np, dtype) as sim:
+        sim.upload_particles(*[st.p[m] for m in states.MEMBERS], st.cell)
+        sim.step(dict(k, energy_interval=0), STEPS)
+        assert sim.device_errors() == 0
+        got = sim.download_particles()
+        f = sim.download_fields()
+    h, _ = st.run(states.consts_vector(k), STEPS)
+    top = np.bincount(got["cell"]).max()
+    print(f"{np.dtype(dtype).name}: largest cell count {counts0} at upload, {top} after "
+          f"{STEPS} steps")
+    assert top > 64, top  # past the columns sized at upload
+    assert not np.any(f)
+    g = states.canonical(got)
+    o = states.canonical(states.oracle_particles(st))
+    for m in states.MEMBERS + ("cell",):
+        assert np.array_equal(g[m], o[m]), m
