@@ -533,7 +533,7 @@ int note_occupancy(cpic_context* c) {
   return CPIC_OK;
 }
 
-int regrow_columns(cpic_context* c) {
+int regrow_columns(cpic_context* c) try {
   const int64_t n = c->np;
   const size_t rs = c->rsz();
   std::vector<unsigned char> m(7 * rs * (size_t)n);
@@ -545,6 +545,8 @@ int regrow_columns(cpic_context* c) {
     return s;
   c->min_rows = std::max<long>(c->min_rows, (long)c->D * 3 / 2);
   return cpic_upload_particles(c, n, d[0], d[1], d[2], d[3], d[4], d[5], d[6], cell.data());
+} catch (const std::bad_alloc&) {
+  return fail(CPIC_ERR_INVALID, "out of host memory while growing the particle columns");
 }
 
 int check_headroom(cpic_context* c) {
@@ -680,9 +682,10 @@ int cpic_get_stream(cpic_context* c, void** stream) {
   return CPIC_OK;
 }
 
-int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, const void* z,
-                          const void* ux, const void* uy, const void* uz, const void* w,
-                          const int32_t* cell) {
+namespace {
+int upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, const void* z,
+                     const void* ux, const void* uy, const void* uz, const void* w,
+                     const int32_t* cell) {
   if (int s = check_ctx(c)) return s;
   if (n < 0 || n > c->cap)
     return fail(CPIC_ERR_INVALID, "particle count %lld exceeds capacity %ld", (long long)n, c->cap);
@@ -745,8 +748,8 @@ int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void*
   return CPIC_OK;
 }
 
-int cpic_download_particles(cpic_context* c, int64_t n, void* x, void* y, void* z, void* ux,
-                            void* uy, void* uz, void* w, int32_t* cell) {
+int download_particles(cpic_context* c, int64_t n, void* x, void* y, void* z, void* ux,
+                       void* uy, void* uz, void* w, int32_t* cell) {
   if (int s = check_ctx(c)) return s;
   if (n != c->np)
     return fail(CPIC_ERR_INVALID, "download of %lld particles, %ld resident", (long long)n, c->np);
@@ -790,6 +793,29 @@ int cpic_download_particles(cpic_context* c, int64_t n, void* x, void* y, void* 
     }
   }
   return CPIC_OK;
+}
+
+}  // namespace
+
+// The host-side binning allocates buffers of the columns' size: an
+// allocation failure is returned as an error, never thrown across the ABI.
+int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, const void* z,
+                          const void* ux, const void* uy, const void* uz, const void* w,
+                          const int32_t* cell) {
+  try {
+    return upload_particles(c, n, x, y, z, ux, uy, uz, w, cell);
+  } catch (const std::bad_alloc&) {
+    return fail(CPIC_ERR_INVALID, "out of host memory for %lld particles", (long long)n);
+  }
+}
+
+int cpic_download_particles(cpic_context* c, int64_t n, void* x, void* y, void* z, void* ux,
+                            void* uy, void* uz, void* w, int32_t* cell) {
+  try {
+    return download_particles(c, n, x, y, z, ux, uy, uz, w, cell);
+  } catch (const std::bad_alloc&) {
+    return fail(CPIC_ERR_INVALID, "out of host memory for %lld particles", (long long)n);
+  }
 }
 
 int cpic_num_particles(cpic_context* c, int64_t* n) {
