@@ -217,3 +217,36 @@ def test_reference_dumps(real, tmp_path):
         assert b.startswith(f"{step}\n") and len(rows) == 32
         xs = np.array([float(r_[0]) for r_ in rows])
         assert np.allclose(xs, (np.arange(1, 33) - 0.5) * k["dx"], rtol=1e-6)
+
+
+def test_column_deck_host_driver_matches_python_deck(tmp_path):
+    """The electron-column deck (192 x 192 x 64 cells, 32 per cell, 1000
+    steps) through the C++ driver, whose Plasma_Initializer and
+    Column_Field_Initializer call the library's loaders: it exits 0, its
+    Gauss residual drifts by less than 1e-4 of max rho / eps0 over the run,
+    and its energies.txt (6 significant digits) equals the energy history
+    of the Python deck's fused run on the same library."""
+    from cabanapic_amd import Simulation, decks
+    r = run("dioctron_3d", "float", tmp_path, "--chunk", "1000", timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    gauss = [l for l in r.stderr.splitlines() if l.startswith("#Gauss residual")]
+    assert len(gauss) == 1, r.stderr[-2000:]
+    rel = float(gauss[0].split("relative ")[1].rstrip(")"))
+    print(gauss[0])
+    assert rel < 1e-4
+    h = np.loadtxt(tmp_path / "energies.txt")
+    d = decks.dioctron_3d(np.float32)
+    assert h.shape == (d.num_steps, 4) and np.all(np.isfinite(h))
+    parts = d.load_particles()
+    with Simulation(d.nx, d.ny, d.nz, d.num_particles, d.R) as sim:
+        sim.upload_particles(*[parts[m] for m in ("x", "y", "z", "ux", "uy", "uz", "w")],
+                             parts["cell"])
+        del parts
+        sim.upload_fields(d.load_fields())
+        py = sim.step(d.step_params(energy_interval=1), d.num_steps)
+        assert sim.device_errors() == 0
+    for col, what in ((2, "E"), (3, "B")):
+        ref = py[:, col - 2].astype(np.float32).astype(np.float64)
+        err = np.abs(h[:, col] - ref) / np.abs(ref)
+        print(f"{what}: max rel. difference to the Python deck {err.max():.2e}")
+        assert err.max() < 1e-5, what
