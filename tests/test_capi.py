@@ -85,3 +85,34 @@ def test_invalid_config_rejected_before_any_device_call():
     assert lib.cpic_create(ctypes.byref(cfg), ctypes.byref(h)) == _capi.CPIC_ERR_UNSUPPORTED
     with pytest.raises(_capi.CpicError):
         _capi.check(lib.cpic_destroy(None) or _capi.CPIC_ERR_INVALID)
+
+
+def test_column_loaders_reject_invalid_specs():
+    """cpic_init_plasma and cpic_init_column_fields (host-side loaders, no
+    device call) refuse an unknown profile, a column without a ring width or
+    guide field, and a field request for a uniform plasma."""
+    import numpy as np
+    lib = _capi.load()
+
+    def spec(**kw):
+        s = _capi.PlasmaSpec()
+        s.seed, s.nx, s.ny, s.nz, s.nppc, s.weight = 3, 8, 8, 2, 2, 1.0
+        s.profile, s.len_x, s.len_y = 1, 0.8, 0.8
+        s.ring_radius, s.ring_width, s.column_charge, s.guide_bz = 0.24, 0.04, -0.25, 1.0
+        for k, v in kw.items():
+            setattr(s, k, v)
+        return s
+
+    n = 8 * 8 * 2 * 2
+    arrs = [np.zeros(n) for _ in range(7)]
+    cell = np.zeros(n, np.int32)
+    ptrs = [a.ctypes.data_as(ctypes.c_void_p) for a in arrs] + [cell.ctypes.data_as(ctypes.c_void_p)]
+    f = np.zeros((9, 10 * 10 * 4))
+    fp = f.ctypes.data_as(ctypes.c_void_p)
+    assert lib.cpic_init_plasma(ctypes.byref(spec()), 8, n, *ptrs) == _capi.CPIC_OK
+    assert lib.cpic_init_column_fields(ctypes.byref(spec()), 8, fp) == _capi.CPIC_OK
+    assert np.all(f[5] == 1.0) and np.any(f[0]) and np.any(f[1])
+    for bad in (dict(profile=2), dict(ring_width=0.0), dict(guide_bz=0.0), dict(len_y=0.0)):
+        assert lib.cpic_init_plasma(ctypes.byref(spec(**bad)), 8, n, *ptrs) == _capi.CPIC_ERR_INVALID, bad
+    assert lib.cpic_init_column_fields(ctypes.byref(spec(profile=0)), 8, fp) == _capi.CPIC_ERR_INVALID
+    assert lib.cpic_init_column_fields(ctypes.byref(spec()), 3, fp) == _capi.CPIC_ERR_INVALID
