@@ -34,6 +34,20 @@ def lib():
     return _lib
 
 
+def build_planted(define, outdir):
+    """Builds the oracle with one ORA_PLANT_* switch on (a planted mistake)
+    into outdir and returns it loaded.  Used only to show that the pinning
+    checks fail on a wrong oracle."""
+    out = os.path.join(outdir, f"liboracle_{define}.so")
+    srcs = [os.path.join(ORACLE_DIR, f) for f in
+            ("oracle_float.c", "oracle_double.c", "oracle_smooth.c")]
+    subprocess.run(["gcc", "-O1", "-std=c11", "-ffp-contract=off", "-fno-fast-math", "-fopenmp",
+                    "-fPIC", "-shared", f"-D{define}", "-o", out] + srcs + ["-lm"], check=True)
+    L = ctypes.CDLL(out)
+    _declare(L)
+    return L
+
+
 def _declare(L):
     P = ctypes.c_void_p
     Lg = ctypes.c_long
@@ -101,12 +115,14 @@ class OracleState:
         return [_p(self.p[k]) for k in ("x", "y", "z", "ux", "uy", "uz", "w")] + [_p(self.cell)]
 
     def fn(self, name):
+        # library: the oracle, or a planted variant (build_planted)
+        L = getattr(self, "library", None) or lib()
         # smooth: the FP64 oracle with a double sqrt in place of the
         # reference's sqrtf (oracle/oracle_smooth.c), for finite differences
         if getattr(self, "smooth", False):
             assert self.dtype == np.float64
-            return getattr(lib(), name + "_ds")
-        return getattr(lib(), name + sfx(self.dtype))
+            return getattr(L, name + "_ds")
+        return getattr(L, name + sfx(self.dtype))
 
     def load_interpolator(self):
         self.fn("ora_load_interpolator")(_p(self.fields), _p(self.interp), self.nx, self.ny, self.nz)
@@ -201,9 +217,10 @@ def default_particle_init(st, dxp, nppc, we, v0):
     st.fn("ora_default_particle_init")(*st._pp(), st.np, st.nx, st.ny, dxp, nppc, we, v0)
 
 
-def setup_2stream_em(dtype):
+def setup_2stream_em(dtype, library=None):
     """Initial state of the 2stream-em deck: zero fields, default particles."""
     d = deck_2stream_em(dtype)
     st = OracleState(d["nx"], d["ny"], d["nz"], d["np"], dtype)
+    st.library = library
     default_particle_init(st, d["dxp"], d["nppc"], d["we"], d["v0"])
     return d, st
