@@ -129,6 +129,45 @@ def differentiation_config_costs(steps):
     return 1e3 * t_fwd / steps, 1e3 * t_tan / steps
 
 
+def cpu_description():
+    """CPU model and physical core count of this host (from /proc/cpuinfo),
+    and the cores this process may run on."""
+    model, cores = "unknown", set()
+    phys = core = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                k, _, v = line.partition(":")
+                k, v = k.strip(), v.strip()
+                if k == "model name" and model == "unknown":
+                    model = v
+                elif k == "physical id":
+                    phys = v
+                elif k == "core id":
+                    core = v
+                elif not k and phys is not None and core is not None:
+                    cores.add((phys, core))
+                    phys = core = None
+        if phys is not None and core is not None:
+            cores.add((phys, core))
+    except OSError:
+        pass
+    allowed = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return model, len(cores) or (os.cpu_count() or 1), allowed
+
+
+def cpu_threads():
+    """Threads of the CPU baseline: one per physical core the process may
+    use, at most OMP_NUM_THREADS where the environment sets it (the CPU share
+    of one GPU on a shared host)."""
+    _, phys, allowed = cpu_description()
+    n = min(phys, allowed)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(n, 1)
+
+
 def cpu_baseline(deck, nsteps, nthreads):
     """The oracle's OpenMP step on the full workload for nsteps steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -162,7 +201,9 @@ def main():
                     help="steps of the FP64 path for the FP32/FP64 ratio (0 = skip)")
     ap.add_argument("--cpu-steps", type=int, default=30,
                     help="steps of the CPU baseline (0 = skip)")
-    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0 = one per physical core the process "
+                         "may use, within OMP_NUM_THREADS)")
     ap.add_argument("--tangent-steps", type=int, default=5,
                     help="timed steps of the tangent-linear step (0 = skip)")
     ap.add_argument("--column-steps", type=int, default=20,
@@ -172,6 +213,10 @@ def main():
                     help="only run a plain 1 GiB device-to-device copy (torch), the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
     args = ap.parse_args()
+    # thread placement of the CPU baseline: set before the OpenMP runtime
+    # starts (the oracle library loads it)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
 
     if args.copy_calibration:
         import torch
@@ -265,12 +310,24 @@ def main():
 
     cpu = None
     if args.cpu_steps > 0:
-        nth = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        nth = args.cpu_threads or cpu_threads()
+        model, phys, allowed = cpu_description()
         v_cpu, t_cpu = cpu_baseline(d32, args.cpu_steps, nth)
+        n64 = max(1, args.cpu_steps // 2)
+        v_cpu64, t_cpu64 = cpu_baseline(decks.weibel_3d(np.float64, n=args.n, nppc=args.nppc),
+                                         n64, nth)
         cpu = {"value": v_cpu, "unit": "particle-steps/s", "cores": nth, "kind": "port",
-               "sample": f"oracle OpenMP step (C restatement, FP32) on the full "
-                         f"{d32.name} workload, {args.cpu_steps} steps, {t_cpu:.1f} s"}
-        print(f"# cpu: {v_cpu:.4g} particle-steps/s on {nth} threads", file=sys.stderr,
+               "sample": f"oracle OpenMP step (C restatement of the reference step, not the "
+                         f"Kokkos build; per-thread accumulator copies summed after the push) "
+                         f"on the full {d32.name} workload, FP32, {args.cpu_steps} steps, "
+                         f"{t_cpu:.1f} s",
+               "fp64_value": v_cpu64,
+               "fp64_sample": f"same, FP64, {n64} steps, {t_cpu64:.1f} s",
+               "cpu_model": model, "physical_cores": phys, "cores_allowed": allowed,
+               "binding": f"OMP_PROC_BIND={os.environ.get('OMP_PROC_BIND')} "
+                          f"OMP_PLACES={os.environ.get('OMP_PLACES')}"}
+        print(f"# cpu ({model}, {phys} physical cores, {allowed} allowed): FP32 {v_cpu:.4g}, "
+              f"FP64 {v_cpu64:.4g} particle-steps/s on {nth} threads", file=sys.stderr,
               flush=True)
 
     line = {

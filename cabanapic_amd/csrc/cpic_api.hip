@@ -46,6 +46,21 @@ int fail(int code, const char* fmt, ...) {
 
 inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Runs an entry point's body; a host allocation failure (or any other C++
+// exception) becomes an error code instead of crossing the C ABI.
+template <class F>
+int guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::bad_alloc&) {
+    return fail(CPIC_ERR_INVALID, "out of host memory");
+  } catch (const std::exception& e) {
+    return fail(CPIC_ERR_INVALID, "host error: %s", e.what());
+  } catch (...) {
+    return fail(CPIC_ERR_INVALID, "host error");
+  }
+}
+
 // Pushes between column compactions when the caller names none.  The in-place
 // pass fills the holes departures leave before it lengthens a column, and
 // rows past the ends of all the columns of a line cost it nothing, so packing
@@ -55,31 +70,14 @@ constexpr int kCompactInterval = 32;
 
 }  // namespace
 
-struct cpic_context {
-  int real_size = 4;
-  long nx = 0, ny = 0, nz = 0;
-  long ncells = 0;  // Ng
-  long nc = 0;      // interior cells
-  long nc_pad = 0;  // interior cells rounded up to whole blocks (kBlock lanes)
-  long cap = 0;     // particle capacity declared at creation
-  long column_rows = 0;  // requested rows per column (0 = automatic)
-  long min_rows = 0;     // rows per column the next upload keeps at least (regrowth)
-  int* occ_host = nullptr;        // pinned copy of the last push's max_count
-  hipEvent_t occ_event = nullptr;  // recorded after that copy
-  bool occ_pending = false;        // a copy is in flight
-  long np = 0;      // particles resident (conserved by the push)
-  Grid grid{};
-  CellGrid cg{};
-
-  // particle columns (double-buffered), see cpic_push_kernels.h
+// The particle columns and everything sized by them (see
+// cpic_push_kernels.h): replaced as a whole when the capacity changes.
+struct ColumnSet {
   int D = 0;               // rows per column
-  void* cols[2] = {};      // kSlotLen reals per slot: x,y,z,ux,uy,uz,w,pad
+  int MW = 0;              // mask words per cell
+  void* cols[2] = {};      // kSlotLen reals per slot: x,y,z,ux,uy,uz,w,tag
   int* cnt[2] = {};        // rows in use per interior cell
   unsigned long long* mask[2] = {};  // occupied rows, [MW][nc_pad]
-  int MW = 0;              // mask words per cell
-  int cur = 0;             // buffer holding the current particles
-  long pushes = 0;         // pushes since the last upload (the first of every
-                           // compaction interval packs the columns)
   // crossing records: list 1 (first and third crossings), list 2 (second)
   int M1 = 0, M2 = 0;
   void* rec1 = nullptr;
@@ -88,11 +86,32 @@ struct cpic_context {
   int* rc2 = nullptr;
   void* ovf_rec = nullptr;
   int* ovf_cell = nullptr;
-  unsigned int* ovf_count = nullptr;
   int ovf_cap = 0;
-  int* max_count = nullptr;  // largest column occupancy seen by the last push
   unsigned long long* dep = nullptr;  // departure masks of pass 1, [7][MW][nc_pad]
   int* dep_words = nullptr;           // mask words written by pass 1, per group
+};
+
+struct cpic_context : ColumnSet {
+  int real_size = 4;
+  long nx = 0, ny = 0, nz = 0;
+  long ncells = 0;  // Ng
+  long nc = 0;      // interior cells
+  long nc_pad = 0;  // interior cells rounded up to whole blocks (kBlock lanes)
+  long cap = 0;     // particle capacity declared at creation
+  long column_rows = 0;  // requested rows per column (0 = automatic)
+  int* occ_host = nullptr;        // pinned copies of the last push's max_count and error flags
+  hipEvent_t occ_event = nullptr;  // recorded after those copies
+  bool occ_pending = false;        // a copy is in flight
+  long np = 0;      // particles resident (conserved by the push)
+  Grid grid{};
+  CellGrid cg{};
+
+  // particle columns (double-buffered): the ColumnSet base
+  int cur = 0;             // buffer holding the current particles
+  long pushes = 0;         // pushes since the last upload (the first of every
+                           // compaction interval packs the columns)
+  unsigned int* ovf_count = nullptr;
+  int* max_count = nullptr;  // largest column occupancy seen by the last push
 
   void* fields = nullptr;
   void* interp = nullptr;
@@ -250,7 +269,8 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   StageTimer t(c, CPIC_STAGE_PUSH);
   int s;
   if ((s = fill_zero<unsigned int>(c, c->movers, 1))) return s;
-  if (c->np == 0 || c->D == 0) return t.finish();
+  if (c->np == 0) return t.finish();
+  if (c->D == 0) return fail(CPIC_ERR_INVALID, "particles resident without columns");
   if ((s = fill_zero<unsigned int>(c, c->ovf_count, 1))) return s;
   if ((s = fill_zero<int>(c, c->max_count, 1))) return s;
   const int blocks = cdiv(c->nc, kBlock);
@@ -438,62 +458,86 @@ int launch_kinetic(cpic_context* c, double scale, double* out) {
   return CPIC_OK;
 }
 
-// (Re)allocates the two particle column buffers (D rows per cell) and the
-// crossing record lists.  Capacities: D = max(2.5 x, +48) the largest column
-// at upload; M1 record slots per (cell, face) for the common first
-// crossings, M2 for the rarer later ones; spills beyond them are finished by
-// the fallback kernel.
-int ensure_columns(cpic_context* c, int D, int M1, int M2) {
-  if (c->D == D && c->M1 == M1 && c->M2 == M2 && c->cnt[0]) return CPIC_OK;
-  HIP_TRY(hipStreamSynchronize(c->stream));
+// ---- particle columns --------------------------------------------------------
+// Capacities: D rows per column (see upload_particles); M1 record slots per
+// (cell, face) for the common first crossings, M2 for the rarer later ones;
+// spills beyond them are finished by the fallback kernel.
+
+void free_column_set(ColumnSet& s) {
+  // best effort: a free that fails leaves nothing to act upon
   for (int b = 0; b < 2; ++b) {
-    if (c->cols[b]) HIP_TRY(hipFree(c->cols[b]));
-    c->cols[b] = nullptr;
-    if (c->cnt[b]) HIP_TRY(hipFree(c->cnt[b]));
-    c->cnt[b] = nullptr;
-    if (c->mask[b]) HIP_TRY(hipFree(c->mask[b]));
-    c->mask[b] = nullptr;
+    if (s.cols[b]) (void)hipFree(s.cols[b]);
+    if (s.cnt[b]) (void)hipFree(s.cnt[b]);
+    if (s.mask[b]) (void)hipFree(s.mask[b]);
   }
-  void** recs[] = {&c->rec1, &c->rec2, (void**)&c->rc1, (void**)&c->rc2, &c->ovf_rec,
-                   (void**)&c->ovf_cell, (void**)&c->dep, (void**)&c->dep_words};
-  for (void** p : recs) {
-    if (*p) HIP_TRY(hipFree(*p));
-    *p = nullptr;
-  }
-  c->D = 0;
+  void* ptrs[] = {s.rec1, s.rec2, s.rc1, s.rc2, s.ovf_rec, s.ovf_cell, s.dep, s.dep_words};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  s = ColumnSet{};
+}
+
+// Allocates and clears a complete column set for D rows per column into
+// out; on failure frees what it allocated and leaves out empty.  The
+// context's current set is not touched.
+int alloc_column_set(cpic_context* c, int D, int M1, int M2, ColumnSet& out) {
+  ColumnSet s;
+  s.D = D;
+  s.MW = (D + 63) / 64;
+  s.M1 = M1;
+  s.M2 = M2;
+  s.ovf_cap = (int)std::max(65536L, c->nc / 4);
   const size_t len = (size_t)c->nc_pad * (size_t)D;
-  int s;
-  for (int b = 0; b < 2; ++b) {
-    if ((s = alloc(&c->cols[b], c->rsz() * kSlotLen * len))) return s;
-    if ((s = alloc((void**)&c->cnt[b], sizeof(int) * c->nc_pad))) return s;
-    if ((s = alloc((void**)&c->mask[b], sizeof(unsigned long long) * c->nc_pad * ((D + 63) / 64))))
-      return s;
-  }
+  const size_t words = (size_t)s.MW * c->nc_pad;
   const size_t per = (size_t)record_bytes(c->real_size) * 6 * (size_t)c->nc;
-  if ((s = alloc(&c->rec1, per * M1))) return s;
-  if ((s = alloc(&c->rec2, per * M2))) return s;
-  if ((s = alloc((void**)&c->rc1, sizeof(int) * 6 * c->nc))) return s;
-  if ((s = alloc((void**)&c->rc2, sizeof(int) * 6 * c->nc))) return s;
-  c->ovf_cap = (int)std::max(65536L, c->nc / 4);
-  if ((s = alloc(&c->ovf_rec, (size_t)record_bytes(c->real_size) * c->ovf_cap))) return s;
-  if ((s = alloc((void**)&c->ovf_cell, sizeof(int) * c->ovf_cap))) return s;
-  if ((s = fill_zero<int>(c, c->cnt[0], c->nc_pad))) return s;
-  if ((s = fill_zero<int>(c, c->cnt[1], c->nc_pad))) return s;
-  const int MW = (D + 63) / 64;
-  if ((s = alloc((void**)&c->dep,
-                 sizeof(unsigned long long) * kDepMasks * (size_t)MW * c->nc_pad)))
-    return s;
-  if ((s = fill_zero<unsigned long long>(c, c->dep, kDepMasks * (size_t)MW * c->nc_pad))) return s;
-  if ((s = alloc((void**)&c->dep_words, sizeof(int) * (c->nc_pad / 64)))) return s;
-  if ((s = fill_zero<int>(c, c->dep_words, c->nc_pad / 64))) return s;
-  if ((s = fill_zero<unsigned long long>(c, c->mask[0], (size_t)MW * c->nc_pad))) return s;
-  if ((s = fill_zero<unsigned long long>(c, c->mask[1], (size_t)MW * c->nc_pad))) return s;
-  c->D = D;
-  c->MW = MW;
-  c->M1 = M1;
-  c->M2 = M2;
+  auto run = [&]() -> int {
+    int t;
+    for (int b = 0; b < 2; ++b) {
+      if ((t = alloc(&s.cols[b], c->rsz() * kSlotLen * len))) return t;
+      if ((t = alloc((void**)&s.cnt[b], sizeof(int) * c->nc_pad))) return t;
+      if ((t = alloc((void**)&s.mask[b], sizeof(unsigned long long) * words))) return t;
+    }
+    if ((t = alloc(&s.rec1, per * M1))) return t;
+    if ((t = alloc(&s.rec2, per * M2))) return t;
+    if ((t = alloc((void**)&s.rc1, sizeof(int) * 6 * c->nc))) return t;
+    if ((t = alloc((void**)&s.rc2, sizeof(int) * 6 * c->nc))) return t;
+    if ((t = alloc(&s.ovf_rec, (size_t)record_bytes(c->real_size) * s.ovf_cap))) return t;
+    if ((t = alloc((void**)&s.ovf_cell, sizeof(int) * s.ovf_cap))) return t;
+    if ((t = alloc((void**)&s.dep, sizeof(unsigned long long) * kDepMasks * words))) return t;
+    if ((t = alloc((void**)&s.dep_words, sizeof(int) * (c->nc_pad / 64)))) return t;
+    for (int b = 0; b < 2; ++b) {
+      if ((t = fill_zero<int>(c, s.cnt[b], c->nc_pad))) return t;
+      if ((t = fill_zero<unsigned long long>(c, s.mask[b], words))) return t;
+    }
+    if ((t = fill_zero<unsigned long long>(c, s.dep, kDepMasks * words))) return t;
+    if ((t = fill_zero<int>(c, s.dep_words, c->nc_pad / 64))) return t;
+    return CPIC_OK;
+  };
+  if (int t = run()) {
+    (void)hipStreamSynchronize(c->stream);
+    free_column_set(s);
+    return t;
+  }
+  out = s;
   return CPIC_OK;
 }
+
+// Replaces the context's column set by s (after the stream has drained).
+int install_column_set(cpic_context* c, ColumnSet& s) {
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  ColumnSet& mine = *c;
+  free_column_set(mine);
+  mine = s;
+  s = ColumnSet{};
+  return CPIC_OK;
+}
+
+// Device scratch of a synchronous host transfer, freed on every path.
+struct Scratch {
+  void* p = nullptr;
+  ~Scratch() {
+    if (p) (void)hipFree(p);
+  }
+};
 
 int check_periodic(int boundary) {
   if (boundary != CPIC_BOUNDARY_PERIODIC)
@@ -512,49 +556,64 @@ int check_device_errors(cpic_context* c) {
 }
 
 // ---- column headroom -------------------------------------------------------
-// Each push copies its largest input column occupancy (rows in use) to
-// pinned host memory.  Before the next push the host reads the copy; a
+// Each push copies its largest input column occupancy (rows in use) and the
+// device error flags to pinned host memory.  Before the next push the host
+// reads the copies (one step behind, so the GPU does not idle).  A set flag
+// fails the push: the kernels of a push also return at once when a flag is
+// set, so pushes already queued behind a failed one change nothing.  A
 // column within max(8, D/4) rows of its capacity makes the host re-lay the
-// columns out (download, then upload with at least 1.5x the rows), so that
-// particles bunching in one cell (a nonlinear two-stream phase reaches 4x
-// the mean count) do not overflow it.  A push that overflows anyway still
-// fails with kErrColumnFull.
+// columns out on the device with 1.5x the rows, so that particles bunching
+// in one cell (a nonlinear two-stream phase reaches 4x the mean count) do
+// not overflow it.  A push that overflows anyway fails with kErrColumnFull.
 
 int note_occupancy(cpic_context* c) {
   if (c->D == 0 || c->np == 0) return CPIC_OK;
   if (!c->occ_host) {
-    HIP_TRY(hipHostMalloc((void**)&c->occ_host, sizeof(int), hipHostMallocDefault));
+    HIP_TRY(hipHostMalloc((void**)&c->occ_host, 2 * sizeof(int), hipHostMallocDefault));
     HIP_TRY(hipEventCreateWithFlags(&c->occ_event, hipEventDisableTiming));
   }
   HIP_TRY(hipMemcpyAsync(c->occ_host, c->max_count, sizeof(int), hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipMemcpyAsync(c->occ_host + 1, c->err, sizeof(int), hipMemcpyDeviceToHost,
                          c->stream));
   HIP_TRY(hipEventRecord(c->occ_event, c->stream));
   c->occ_pending = true;
   return CPIC_OK;
 }
 
-int regrow_columns(cpic_context* c) try {
-  const int64_t n = c->np;
-  const size_t rs = c->rsz();
-  std::vector<unsigned char> m(7 * rs * (size_t)n);
-  std::vector<int32_t> cell((size_t)n);
-  void* d[7];
-  for (int k = 0; k < 7; ++k) d[k] = m.data() + (size_t)k * rs * (size_t)n;
-  if (int s = cpic_download_particles(c, n, d[0], d[1], d[2], d[3], d[4], d[5], d[6],
-                                      cell.data()))
-    return s;
-  c->min_rows = std::max<long>(c->min_rows, (long)c->D * 3 / 2);
-  return cpic_upload_particles(c, n, d[0], d[1], d[2], d[3], d[4], d[5], d[6], cell.data());
-} catch (const std::bad_alloc&) {
-  return fail(CPIC_ERR_INVALID, "out of host memory while growing the particle columns");
+// The particles packed into columns of 1.5x the rows, on the device.
+template <class R>
+int regrow_columns(cpic_context* c) {
+  const int D = (c->D * 3 / 2 + 63) / 64 * 64;
+  ColumnSet s;
+  if (int t = alloc_column_set(c, D, c->M1, c->M2, s)) return t;
+  const int blocks = cdiv(c->nc_pad, kBlock);
+  k_relayout<R><<<blocks, kBlock, 0, c->stream>>>(
+      (const R*)c->cols[c->cur], c->cnt[c->cur], c->mask[c->cur], c->D, (R*)s.cols[0], s.cnt[0],
+      D, (int)c->nc, (int)c->nc_pad, c->err);
+  k_masks_packed<<<blocks, kBlock, 0, c->stream>>>(s.cnt[0], s.mask[0], (int)c->nc,
+                                                   (int)c->nc_pad, s.MW);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    (void)hipStreamSynchronize(c->stream);
+    free_column_set(s);
+    return fail(CPIC_ERR_HIP, "column regrowth launch failed: %s", hipGetErrorString(e));
+  }
+  if (int t = install_column_set(c, s)) return t;
+  c->cur = 0;
+  c->pushes = 0;
+  return CPIC_OK;
 }
 
 int check_headroom(cpic_context* c) {
   if (!c->occ_pending) return CPIC_OK;
   HIP_TRY(hipEventSynchronize(c->occ_event));
   c->occ_pending = false;
-  if (c->D == 0 || *c->occ_host < c->D - std::max(8, c->D / 4)) return CPIC_OK;
-  return regrow_columns(c);
+  if (c->occ_host[1])
+    return fail(CPIC_ERR_DEVICE, "an earlier push failed on the device (flags=%d)",
+                c->occ_host[1]);
+  if (c->D == 0 || c->occ_host[0] < c->D - std::max(8, c->D / 4)) return CPIC_OK;
+  return with_real(c, [&](auto r) { return regrow_columns<decltype(r)>(c); });
 }
 
 }  // namespace
@@ -566,123 +625,146 @@ extern "C" {
 const char* cpic_last_error(void) { return g_last_error.c_str(); }
 
 int cpic_create(const cpic_config* cfg, cpic_context** out) {
-  if (!cfg || !out) return fail(CPIC_ERR_INVALID, "null argument");
-  *out = nullptr;
-  if (cfg->real_size != 4 && cfg->real_size != 8)
-    return fail(CPIC_ERR_INVALID, "real_size must be 4 or 8 (got %d)", cfg->real_size);
-  if (cfg->num_ghosts != 1)
-    return fail(CPIC_ERR_UNSUPPORTED, "num_ghosts must be 1 (got %d)", cfg->num_ghosts);
-  if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1)
-    return fail(CPIC_ERR_INVALID, "grid extents must be >= 1");
-  const long X = cfg->nx + 2, Y = cfg->ny + 2, Z = cfg->nz + 2;
-  if ((double)X * Y * Z > 2147483647.0 / 20)
-    return fail(CPIC_ERR_INVALID, "grid too large for 32-bit voxel indexing");
-  if (cfg->max_particles < 0 || cfg->max_particles > 2147483647L)
-    return fail(CPIC_ERR_INVALID, "max_particles out of range");
-  if (cfg->column_rows < 0 || cfg->column_rows > (1L << 20))
-    return fail(CPIC_ERR_INVALID, "column_rows out of range [0, 2^20]");
-
-  cpic_context* c = new (std::nothrow) cpic_context();
-  if (!c) return fail(CPIC_ERR_INVALID, "out of host memory");
-  c->real_size = cfg->real_size;
-  c->nx = cfg->nx;
-  c->ny = cfg->ny;
-  c->nz = cfg->nz;
-  c->ncells = X * Y * Z;
-  c->cap = cfg->max_particles;
-  c->column_rows = (long)cfg->column_rows;
-  c->grid.nx = (int)c->nx;
-  c->grid.ny = (int)c->ny;
-  c->grid.nz = (int)c->nz;
-  c->grid.sy = (int)X;
-  c->grid.sz = (int)(X * Y);
-  c->grid.ncells = (int)c->ncells;
-  c->nc = cfg->nx * cfg->ny * cfg->nz;
-  c->nc_pad = (c->nc + kBlock - 1) / kBlock * kBlock;
-  c->cg.nx = (int)c->nx;
-  c->cg.ny = (int)c->ny;
-  c->cg.nz = (int)c->nz;
-  c->cg.nc = (int)c->nc;
-  c->cg.sy = (int)X;
-  c->cg.sz = (int)(X * Y);
-  c->n_partials = 4096;
-
-  auto bail = [&](int s) {
-    cpic_destroy(c);
-    return s;
-  };
-  int s;
-  if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(CPIC_ERR_HIP, "no HIP device"));
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-    return bail(fail(CPIC_ERR_HIP, "hipStreamCreate failed"));
-  if ((s = alloc(&c->fields, c->rsz() * 9 * c->ncells))) return bail(s);
-  if ((s = alloc(&c->interp, c->rsz() * kInterpStride * c->ncells))) return bail(s);
-  if ((s = alloc(&c->acc, c->rsz() * 12 * c->ncells))) return bail(s);
-  if ((s = alloc((void**)&c->partials, sizeof(double) * 3 * c->n_partials))) return bail(s);
-  if ((s = alloc((void**)&c->red_out, sizeof(double) * 4))) return bail(s);
-  if ((s = alloc((void**)&c->err, sizeof(int)))) return bail(s);
-  if ((s = alloc((void**)&c->movers, sizeof(unsigned int)))) return bail(s);
-  if ((s = alloc((void**)&c->ovf_count, sizeof(unsigned int)))) return bail(s);
-  if ((s = alloc((void**)&c->max_count, sizeof(int)))) return bail(s);
-  s = with_real(c, [&](auto r) {
-    using R = decltype(r);
-    int t;
-    if ((t = fill_zero<R>(c, c->fields, 9 * (size_t)c->ncells))) return t;
-    if ((t = fill_zero<R>(c, c->interp, kInterpStride * (size_t)c->ncells))) return t;
-    if ((t = fill_zero<R>(c, c->acc, 12 * (size_t)c->ncells))) return t;
+  return guarded([&]() -> int {
+    if (!cfg || !out) return fail(CPIC_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (cfg->real_size != 4 && cfg->real_size != 8)
+      return fail(CPIC_ERR_INVALID, "real_size must be 4 or 8 (got %d)", cfg->real_size);
+    if (cfg->num_ghosts != 1)
+      return fail(CPIC_ERR_UNSUPPORTED, "num_ghosts must be 1 (got %d)", cfg->num_ghosts);
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1)
+      return fail(CPIC_ERR_INVALID, "grid extents must be >= 1");
+    const long X = cfg->nx + 2, Y = cfg->ny + 2, Z = cfg->nz + 2;
+    if ((double)X * Y * Z > 2147483647.0 / 20)
+      return fail(CPIC_ERR_INVALID, "grid too large for 32-bit voxel indexing");
+    if (cfg->max_particles < 0 || cfg->max_particles > 2147483647L)
+      return fail(CPIC_ERR_INVALID, "max_particles out of range");
+    if (cfg->column_rows < 0 || cfg->column_rows > (1L << 20))
+      return fail(CPIC_ERR_INVALID, "column_rows out of range [0, 2^20]");
+  
+    cpic_context* c = new (std::nothrow) cpic_context();
+    if (!c) return fail(CPIC_ERR_INVALID, "out of host memory");
+    c->real_size = cfg->real_size;
+    c->nx = cfg->nx;
+    c->ny = cfg->ny;
+    c->nz = cfg->nz;
+    c->ncells = X * Y * Z;
+    c->cap = cfg->max_particles;
+    c->column_rows = (long)cfg->column_rows;
+    c->grid.nx = (int)c->nx;
+    c->grid.ny = (int)c->ny;
+    c->grid.nz = (int)c->nz;
+    c->grid.sy = (int)X;
+    c->grid.sz = (int)(X * Y);
+    c->grid.ncells = (int)c->ncells;
+    c->nc = cfg->nx * cfg->ny * cfg->nz;
+    c->nc_pad = (c->nc + kBlock - 1) / kBlock * kBlock;
+    c->cg.nx = (int)c->nx;
+    c->cg.ny = (int)c->ny;
+    c->cg.nz = (int)c->nz;
+    c->cg.nc = (int)c->nc;
+    c->cg.sy = (int)X;
+    c->cg.sz = (int)(X * Y);
+    c->n_partials = 4096;
+  
+    auto bail = [&](int s) {
+      cpic_destroy(c);
+      return s;
+    };
+    int s;
+    if (hipGetDevice(&c->device) != hipSuccess) return bail(fail(CPIC_ERR_HIP, "no HIP device"));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+      return bail(fail(CPIC_ERR_HIP, "hipStreamCreate failed"));
+    if ((s = alloc(&c->fields, c->rsz() * 9 * c->ncells))) return bail(s);
+    if ((s = alloc(&c->interp, c->rsz() * kInterpStride * c->ncells))) return bail(s);
+    if ((s = alloc(&c->acc, c->rsz() * 12 * c->ncells))) return bail(s);
+    if ((s = alloc((void**)&c->partials, sizeof(double) * 3 * c->n_partials))) return bail(s);
+    if ((s = alloc((void**)&c->red_out, sizeof(double) * 4))) return bail(s);
+    if ((s = alloc((void**)&c->err, sizeof(int)))) return bail(s);
+    if ((s = alloc((void**)&c->movers, sizeof(unsigned int)))) return bail(s);
+    if ((s = alloc((void**)&c->ovf_count, sizeof(unsigned int)))) return bail(s);
+    if ((s = alloc((void**)&c->max_count, sizeof(int)))) return bail(s);
+    s = with_real(c, [&](auto r) {
+      using R = decltype(r);
+      int t;
+      if ((t = fill_zero<R>(c, c->fields, 9 * (size_t)c->ncells))) return t;
+      if ((t = fill_zero<R>(c, c->interp, kInterpStride * (size_t)c->ncells))) return t;
+      if ((t = fill_zero<R>(c, c->acc, 12 * (size_t)c->ncells))) return t;
+      return CPIC_OK;
+    });
+    if (s) return bail(s);
+    if (hipMemsetAsync(c->err, 0, sizeof(int), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->movers, 0, sizeof(unsigned int), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess)
+      return bail(fail(CPIC_ERR_HIP, "context initialisation failed"));
+    *out = c;
     return CPIC_OK;
   });
-  if (s) return bail(s);
-  if (hipMemsetAsync(c->err, 0, sizeof(int), c->stream) != hipSuccess ||
-      hipMemsetAsync(c->movers, 0, sizeof(unsigned int), c->stream) != hipSuccess ||
-      hipStreamSynchronize(c->stream) != hipSuccess)
-    return bail(fail(CPIC_ERR_HIP, "context initialisation failed"));
-  *out = c;
-  return CPIC_OK;
 }
 
 int cpic_destroy(cpic_context* c) {
-  if (!c) return CPIC_OK;
-  // teardown is best effort: errors cannot be acted upon here
-  if (c->stream) (void)hipStreamSynchronize(c->stream);
-  for (auto& p : c->pending) {
-    (void)hipEventDestroy(p.a);
-    (void)hipEventDestroy(p.b);
-  }
-  for (auto e : c->free_events) (void)hipEventDestroy(e);
-  for (int b = 0; b < 2; ++b) {
-    if (c->cols[b]) (void)hipFree(c->cols[b]);
-    if (c->cnt[b]) (void)hipFree(c->cnt[b]);
-    if (c->mask[b]) (void)hipFree(c->mask[b]);
-  }
-  void* ptrs[] = {c->rec1,   c->rec2,     c->rc1,      c->rc2,     c->ovf_rec,
-                  c->ovf_cell, c->ovf_count, c->max_count, c->fields, c->interp,
-                  c->acc,    c->partials, c->red_out,  c->err,     c->movers,
-                  c->hist_dev, c->dep, c->dep_words, c->gauss_q, c->gauss_base,
-                  c->gauss_out};
-  for (void* p : ptrs)
-    if (p) (void)hipFree(p);
-  if (c->occ_event) (void)hipEventDestroy(c->occ_event);
-  if (c->occ_host) (void)hipHostFree(c->occ_host);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (!c) return CPIC_OK;
+    // teardown is best effort: errors cannot be acted upon here
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    for (auto& p : c->pending) {
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    for (auto e : c->free_events) (void)hipEventDestroy(e);
+    free_column_set(*c);
+    void* ptrs[] = {c->ovf_count, c->max_count, c->fields,   c->interp,   c->acc,
+                    c->partials,  c->red_out,   c->err,      c->movers,   c->hist_dev,
+                    c->gauss_q,   c->gauss_base, c->gauss_out};
+    for (void* p : ptrs)
+      if (p) (void)hipFree(p);
+    if (c->occ_event) (void)hipEventDestroy(c->occ_event);
+    if (c->occ_host) (void)hipHostFree(c->occ_host);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return CPIC_OK;
+  });
 }
 
 int cpic_synchronize(cpic_context* c) {
-  if (int s = check_ctx(c)) return s;
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  return collect_timing(c);
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return collect_timing(c);
+  });
 }
 
 int cpic_get_stream(cpic_context* c, void** stream) {
-  if (int s = check_ctx(c)) return s;
-  if (!stream) return fail(CPIC_ERR_INVALID, "null stream pointer");
-  *stream = (void*)c->stream;
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!stream) return fail(CPIC_ERR_INVALID, "null stream pointer");
+    *stream = (void*)c->stream;
+    return CPIC_OK;
+  });
 }
 
 namespace {
+
+// Column rows for a largest cell count maxc at upload: 2.5x (at least +48),
+// rounded up to whole 64-row mask words (the masks and departure words are
+// allocated per word, so the rows up to the next word cost only slots); every
+// particle in one cell where that costs at most 64 MB of columns; at least
+// the configured column_rows.
+int column_rows_for(const cpic_context* c, int maxc, int64_t n) {
+  auto up8 = [](long v) { return (int)((v + 7) / 8 * 8); };
+  int D = (std::max({maxc * 5 / 2, maxc + 48, 16}) + 63) / 64 * 64;
+  const double worst_bytes = (double)c->nc_pad * (double)up8(n) * kSlotLen * (double)c->rsz();
+  if (worst_bytes <= 64.0 * 1024 * 1024) D = std::max(D, up8(n));
+  if (c->column_rows > 0) D = std::max(D, up8(c->column_rows));
+  return D;
+}
+
+// Binning on the device: the host validates the cells and computes each
+// particle's slot (its cell's column, rows in input order); the particles
+// and slots go to a device staging area, and a kernel scatters them into
+// freshly cleared columns.  A new column set is allocated before the old one
+// is released, so a failed allocation leaves the resident particles as they
+// were; a failure after that leaves the context without particles.
 int upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, const void* z,
                      const void* ux, const void* uy, const void* uz, const void* w,
                      const int32_t* cell) {
@@ -697,8 +779,8 @@ int upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, c
   }
   // every particle must sit in an interior voxel (src/types.h:40)
   const long X = c->nx + 2, Y = c->ny + 2;
-  std::vector<int> ci(n);
   std::vector<int> count(c->nc_pad, 0);
+  std::vector<long long> slot((size_t)n);  // (cell << 32 | rank), then the slot
   int maxc = 0;
   for (int64_t p = 0; p < n; ++p) {
     const long v = cell[p];
@@ -706,473 +788,572 @@ int upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, c
     if (v < 0 || v >= c->ncells || ix < 1 || ix > c->nx || iy < 1 || iy > c->ny || iz < 1 ||
         iz > c->nz)
       return fail(CPIC_ERR_INVALID, "particle %lld has non-interior cell %ld", (long long)p, v);
-    ci[p] = (int)((ix - 1) + c->nx * ((iy - 1) + c->ny * (iz - 1)));
-    maxc = std::max(maxc, ++count[ci[p]]);
+    const long ci = (ix - 1) + c->nx * ((iy - 1) + c->ny * (iz - 1));
+    slot[p] = ((long long)ci << 32) | (long long)count[ci];
+    maxc = std::max(maxc, ++count[ci]);
   }
+  const int D = column_rows_for(c, maxc, n);
   auto up8 = [](long v) { return (int)((v + 7) / 8 * 8); };
-  // rounded up to whole 64-row mask words: the masks and departure words are
-  // allocated per word, so the rows up to the next word cost only slots
-  int D = (std::max({maxc * 5 / 2, maxc + 48, 16}) + 63) / 64 * 64;
-  // every particle in one cell, where that costs at most 64 MB of columns
-  const double worst_bytes = (double)c->nc_pad * (double)up8(n) * kSlotLen * (double)c->rsz();
-  if (worst_bytes <= 64.0 * 1024 * 1024) D = std::max(D, up8(n));
-  if (c->column_rows > 0) D = std::max(D, up8(c->column_rows));
-  if (c->min_rows > 0) D = std::max(D, up8(c->min_rows));
   const int M1 = std::min(D, std::max(16, up8(maxc / 2)));
   const int M2 = std::min(D, 8);
-  if (int s = ensure_columns(c, D, M1, M2)) return s;
-  // bin on the host, keeping the input order within each cell
-  const size_t rs = c->rsz(), len = c->col_len();
-  std::vector<unsigned char> buf(rs * len * kSlotLen, 0);
-  std::vector<int> fill(c->nc_pad, 0);
   for (int64_t p = 0; p < n; ++p) {
-    const int cc = ci[p];
-    const size_t slot = ((size_t)(cc >> 6) * D + (size_t)fill[cc]++) * 64 + (cc & 63);
-    for (int k = 0; k < 7; ++k)
-      memcpy(&buf[(slot * kSlotLen + k) * rs], (const unsigned char*)src[k] + p * rs, rs);
+    const long long ci = slot[p] >> 32, r = slot[p] & 0xffffffffll;
+    slot[p] = ((ci >> 6) * D + r) * 64 + (ci & 63);
   }
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(c->cols[0], buf.data(), buf.size(), hipMemcpyHostToDevice));
-  HIP_TRY(hipMemcpy(c->cnt[0], count.data(), sizeof(int) * c->nc_pad, hipMemcpyHostToDevice));
-  std::vector<unsigned long long> mask((size_t)c->MW * c->nc_pad, 0ull);
-  for (long i = 0; i < c->nc; ++i)
-    for (int wd = 0; wd < c->MW; ++wd) {
-      const int bits = count[i] - 64 * wd;
-      mask[(size_t)wd * c->nc_pad + i] = bits >= 64 ? ~0ull : (bits <= 0 ? 0ull : ((1ull << bits) - 1));
+  const size_t rs = c->rsz();
+  Scratch stage;
+  if (n > 0)
+    if (int s = alloc(&stage.p, 7 * rs * (size_t)n + sizeof(long long) * (size_t)n)) return s;
+  ColumnSet fresh;
+  const bool realloc = !(c->D == D && c->M1 == M1 && c->M2 == M2 && c->cnt[0]);
+  if (realloc)
+    if (int s = alloc_column_set(c, D, M1, M2, fresh)) return s;
+  char* sp = (char*)stage.p;
+  long long* slot_dev = (long long*)(sp + 7 * rs * (size_t)n);
+  auto stage_in = [&]() -> int {
+    for (int k = 0; k < 7 && n > 0; ++k)
+      HIP_TRY(hipMemcpyAsync(sp + k * rs * (size_t)n, src[k], rs * (size_t)n,
+                             hipMemcpyHostToDevice, c->stream));
+    if (n > 0)
+      HIP_TRY(hipMemcpyAsync(slot_dev, slot.data(), sizeof(long long) * (size_t)n,
+                             hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CPIC_OK;
+  };
+  if (int s = stage_in()) {
+    free_column_set(fresh);
+    return s;
+  }
+  // from here on the resident particles are replaced
+  c->np = 0;
+  if (realloc)
+    if (int s = install_column_set(c, fresh)) return s;
+  int s = with_real(c, [&](auto r) -> int {
+    using R = decltype(r);
+    int t;
+    if ((t = fill_zero<R>(c, c->cols[0], kSlotLen * c->col_len()))) return t;
+    if (n > 0) {
+      const int blocks = (int)std::min<int64_t>((n + kBlock - 1) / kBlock, 65536);
+      k_bin_scatter<R><<<blocks, kBlock, 0, c->stream>>>((const R*)sp, slot_dev, (long)n,
+                                                         (R*)c->cols[0]);
+      CHECK_LAUNCH();
     }
-  HIP_TRY(hipMemcpy(c->mask[0], mask.data(), sizeof(unsigned long long) * mask.size(),
-                    hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(c->cnt[0], count.data(), sizeof(int) * c->nc_pad,
+                           hipMemcpyHostToDevice, c->stream));
+    k_masks_packed<<<cdiv(c->nc_pad, kBlock), kBlock, 0, c->stream>>>(
+        c->cnt[0], c->mask[0], (int)c->nc, (int)c->nc_pad, c->MW);
+    CHECK_LAUNCH();
+    // a fresh particle state: earlier device errors no longer apply
+    if ((t = fill_zero<int>(c, c->err, 1))) return t;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return CPIC_OK;
+  });
+  if (s) return s;
   c->cur = 0;
   c->pushes = 0;
+  c->occ_pending = false;
   c->np = n;
   return CPIC_OK;
 }
 
+// Gathering on the device: per-cell particle counts come to the host, which
+// sets each cell's first output position; a kernel writes the particles
+// (cells in interior order, rows in order) to a staging area that is then
+// downloaded.  tag (may be null) receives each particle's upload index.
 int download_particles(cpic_context* c, int64_t n, void* x, void* y, void* z, void* ux,
-                       void* uy, void* uz, void* w, int32_t* cell) {
+                       void* uy, void* uz, void* w, int32_t* cell, int64_t* tag) {
   if (int s = check_ctx(c)) return s;
   if (n != c->np)
     return fail(CPIC_ERR_INVALID, "download of %lld particles, %ld resident", (long long)n, c->np);
   if (n == 0) return CPIC_OK;
   void* dst[7] = {x, y, z, ux, uy, uz, w};
-  const size_t rs = c->rsz(), len = c->col_len();
-  std::vector<int> count(c->nc_pad);
-  std::vector<unsigned long long> mask((size_t)c->MW * c->nc_pad);
-  std::vector<unsigned char> buf(rs * len * kSlotLen);
+  const size_t rs = c->rsz();
+  const size_t nc = (size_t)c->nc;
+  Scratch stage;
+  const size_t bytes = sizeof(long long) * nc + (7 * rs + sizeof(int) + sizeof(long long)) * n;
+  if (int s = alloc(&stage.p, bytes)) return s;
+  char* sp = (char*)stage.p;
+  long long* first_dev = (long long*)sp;
+  char* soa = sp + sizeof(long long) * nc;
+  int* cell_dev = (int*)(soa + 7 * rs * n);
+  long long* tag_dev = (long long*)(cell_dev + n);
+  int* occ_dev = (int*)soa;  // the counts use the SoA area before the gather
+  std::vector<int> occ(nc);
+  k_column_occupancy<<<cdiv(c->nc, kBlock), kBlock, 0, c->stream>>>(
+      c->cnt[c->cur], c->mask[c->cur], (int)c->nc, (int)c->nc_pad, c->D, occ_dev);
+  CHECK_LAUNCH();
+  HIP_TRY(hipMemcpyAsync(occ.data(), occ_dev, sizeof(int) * nc, hipMemcpyDeviceToHost,
+                         c->stream));
   HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(count.data(), c->cnt[c->cur], sizeof(int) * c->nc_pad, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(mask.data(), c->mask[c->cur], sizeof(unsigned long long) * mask.size(),
-                    hipMemcpyDeviceToHost));
-  auto occupied = [&](long i, int r) {
-    return (mask[(size_t)(r >> 6) * c->nc_pad + i] >> (r & 63)) & 1ull;
-  };
-  long total = 0;
-  for (long i = 0; i < c->nc; ++i)
-    for (int r = 0; r < count[i]; ++r) total += occupied(i, r) ? 1 : 0;
-  if (total != n)
-    return fail(CPIC_ERR_DEVICE, "columns hold %ld particles, %lld expected", total, (long long)n);
-  const long X = c->nx + 2, Y = c->ny + 2;
-  HIP_TRY(hipMemcpy(buf.data(), c->cols[c->cur], buf.size(), hipMemcpyDeviceToHost));
-  for (int k = -1; k < 7; ++k) {
-    if (k >= 0 ? !dst[k] : !cell) continue;
-    // cells in interior order (x fastest), occupied rows in row order
-    long o = 0;
-    for (long i = 0; i < c->nc; ++i) {
-      const long ix = i % c->nx, iy = (i / c->nx) % c->ny, iz = i / (c->nx * c->ny);
-      const int v = (int)((ix + 1) + X * ((iy + 1) + Y * (iz + 1)));
-      for (int r = 0; r < count[i]; ++r) {
-        if (!occupied(i, r)) continue;
-        if (k < 0) {
-          cell[o] = v;
-        } else {
-          const size_t slot = ((size_t)(i >> 6) * c->D + (size_t)r) * 64 + (i & 63);
-          memcpy((unsigned char*)dst[k] + o * rs, &buf[(slot * kSlotLen + k) * rs], rs);
-        }
-        ++o;
-      }
-    }
+  std::vector<long long> first(nc);
+  long long total = 0;
+  for (size_t i = 0; i < nc; ++i) {
+    first[i] = total;
+    total += occ[i];
   }
+  if (total != n)
+    return fail(CPIC_ERR_DEVICE, "columns hold %lld particles, %lld expected", total,
+                (long long)n);
+  HIP_TRY(hipMemcpyAsync(first_dev, first.data(), sizeof(long long) * nc, hipMemcpyHostToDevice,
+                         c->stream));
+  int s = with_real(c, [&](auto r) -> int {
+    using R = decltype(r);
+    k_gather_columns<R><<<cdiv(c->nc, kBlock), kBlock, 0, c->stream>>>(
+        (const R*)c->cols[c->cur], c->cnt[c->cur], c->mask[c->cur], c->cg, c->D,
+        (int)c->nc_pad, first_dev, (long)n, (R*)soa, cell_dev, tag ? tag_dev : nullptr);
+    CHECK_LAUNCH();
+    return CPIC_OK;
+  });
+  if (s) return s;
+  for (int k = 0; k < 7; ++k)
+    if (dst[k])
+      HIP_TRY(hipMemcpyAsync(dst[k], soa + k * rs * n, rs * n, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (cell)
+    HIP_TRY(hipMemcpyAsync(cell, cell_dev, sizeof(int) * n, hipMemcpyDeviceToHost, c->stream));
+  if (tag)
+    HIP_TRY(hipMemcpyAsync(tag, tag_dev, sizeof(long long) * n, hipMemcpyDeviceToHost,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return CPIC_OK;
 }
 
 }  // namespace
 
-// The host-side binning allocates buffers of the columns' size: an
-// allocation failure is returned as an error, never thrown across the ABI.
+// Host allocation failures are returned as errors, never thrown across the ABI.
 int cpic_upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, const void* z,
                           const void* ux, const void* uy, const void* uz, const void* w,
                           const int32_t* cell) {
-  try {
-    return upload_particles(c, n, x, y, z, ux, uy, uz, w, cell);
-  } catch (const std::bad_alloc&) {
-    return fail(CPIC_ERR_INVALID, "out of host memory for %lld particles", (long long)n);
-  }
+  return guarded([&] { return upload_particles(c, n, x, y, z, ux, uy, uz, w, cell); });
 }
 
 int cpic_download_particles(cpic_context* c, int64_t n, void* x, void* y, void* z, void* ux,
                             void* uy, void* uz, void* w, int32_t* cell) {
-  try {
-    return download_particles(c, n, x, y, z, ux, uy, uz, w, cell);
-  } catch (const std::bad_alloc&) {
-    return fail(CPIC_ERR_INVALID, "out of host memory for %lld particles", (long long)n);
-  }
+  return guarded([&] { return download_particles(c, n, x, y, z, ux, uy, uz, w, cell, nullptr); });
+}
+
+int cpic_download_particle_tags(cpic_context* c, int64_t n, int64_t* tag) {
+  if (!tag) return fail(CPIC_ERR_INVALID, "null tag array");
+  return guarded([&] {
+    return download_particles(c, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr, nullptr, tag);
+  });
+}
+
+int cpic_column_rows(cpic_context* c, int64_t* rows) {
+  if (int s = check_ctx(c)) return s;
+  if (!rows) return fail(CPIC_ERR_INVALID, "null pointer");
+  *rows = c->D;
+  return CPIC_OK;
 }
 
 int cpic_num_particles(cpic_context* c, int64_t* n) {
-  if (int s = check_ctx(c)) return s;
-  if (!n) return fail(CPIC_ERR_INVALID, "null pointer");
-  *n = c->np;
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!n) return fail(CPIC_ERR_INVALID, "null pointer");
+    *n = c->np;
+    return CPIC_OK;
+  });
 }
 
 int cpic_upload_fields(cpic_context* c, const void* f) {
-  if (int s = check_ctx(c)) return s;
-  if (!f) return fail(CPIC_ERR_INVALID, "null fields");
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(c->fields, f, c->rsz() * 9 * c->ncells, hipMemcpyHostToDevice));
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!f) return fail(CPIC_ERR_INVALID, "null fields");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->fields, f, c->rsz() * 9 * c->ncells, hipMemcpyHostToDevice));
+    return CPIC_OK;
+  });
 }
 
 int cpic_download_fields(cpic_context* c, void* f) {
-  if (int s = check_ctx(c)) return s;
-  if (!f) return fail(CPIC_ERR_INVALID, "null fields");
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(f, c->fields, c->rsz() * 9 * c->ncells, hipMemcpyDeviceToHost));
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!f) return fail(CPIC_ERR_INVALID, "null fields");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(f, c->fields, c->rsz() * 9 * c->ncells, hipMemcpyDeviceToHost));
+    return CPIC_OK;
+  });
 }
 
 int cpic_upload_interpolators(cpic_context* c, const void* in) {
-  if (int s = check_ctx(c)) return s;
-  if (!in) return fail(CPIC_ERR_INVALID, "null interpolators");
-  // component-major host layout -> cell-major padded device layout
-  const size_t rs = c->rsz();
-  std::vector<unsigned char> buf(rs * kInterpStride * c->ncells, 0);
-  const unsigned char* src = (const unsigned char*)in;
-  for (long i = 0; i < c->ncells; ++i)
-    for (int k = 0; k < 18; ++k)
-      memcpy(&buf[(i * kInterpStride + k) * rs], &src[((size_t)k * c->ncells + i) * rs], rs);
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(c->interp, buf.data(), buf.size(), hipMemcpyHostToDevice));
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!in) return fail(CPIC_ERR_INVALID, "null interpolators");
+    // component-major host layout -> cell-major padded device layout
+    const size_t rs = c->rsz();
+    std::vector<unsigned char> buf(rs * kInterpStride * c->ncells, 0);
+    const unsigned char* src = (const unsigned char*)in;
+    for (long i = 0; i < c->ncells; ++i)
+      for (int k = 0; k < 18; ++k)
+        memcpy(&buf[(i * kInterpStride + k) * rs], &src[((size_t)k * c->ncells + i) * rs], rs);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->interp, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    return CPIC_OK;
+  });
 }
 
 int cpic_download_interpolators(cpic_context* c, void* out) {
-  if (int s = check_ctx(c)) return s;
-  if (!out) return fail(CPIC_ERR_INVALID, "null interpolators");
-  const size_t rs = c->rsz();
-  std::vector<unsigned char> buf(rs * kInterpStride * c->ncells);
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(buf.data(), c->interp, buf.size(), hipMemcpyDeviceToHost));
-  unsigned char* dst = (unsigned char*)out;
-  for (long i = 0; i < c->ncells; ++i)
-    for (int k = 0; k < 18; ++k)
-      memcpy(&dst[((size_t)k * c->ncells + i) * rs], &buf[(i * kInterpStride + k) * rs], rs);
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!out) return fail(CPIC_ERR_INVALID, "null interpolators");
+    const size_t rs = c->rsz();
+    std::vector<unsigned char> buf(rs * kInterpStride * c->ncells);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(buf.data(), c->interp, buf.size(), hipMemcpyDeviceToHost));
+    unsigned char* dst = (unsigned char*)out;
+    for (long i = 0; i < c->ncells; ++i)
+      for (int k = 0; k < 18; ++k)
+        memcpy(&dst[((size_t)k * c->ncells + i) * rs], &buf[(i * kInterpStride + k) * rs], rs);
+    return CPIC_OK;
+  });
 }
 
 int cpic_upload_accumulators(cpic_context* c, const void* in) {
-  if (int s = check_ctx(c)) return s;
-  if (!in) return fail(CPIC_ERR_INVALID, "null accumulators");
-  const size_t rs = c->rsz();
-  std::vector<unsigned char> buf(rs * 12 * c->ncells);
-  const unsigned char* src = (const unsigned char*)in;
-  for (long i = 0; i < c->ncells; ++i)
-    for (int k = 0; k < 12; ++k)
-      memcpy(&buf[((size_t)k * c->ncells + i) * rs], &src[((size_t)i * 12 + k) * rs], rs);
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(c->acc, buf.data(), buf.size(), hipMemcpyHostToDevice));
-  c->acc_ghosts_zero = false;
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!in) return fail(CPIC_ERR_INVALID, "null accumulators");
+    const size_t rs = c->rsz();
+    std::vector<unsigned char> buf(rs * 12 * c->ncells);
+    const unsigned char* src = (const unsigned char*)in;
+    for (long i = 0; i < c->ncells; ++i)
+      for (int k = 0; k < 12; ++k)
+        memcpy(&buf[((size_t)k * c->ncells + i) * rs], &src[((size_t)i * 12 + k) * rs], rs);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(c->acc, buf.data(), buf.size(), hipMemcpyHostToDevice));
+    c->acc_ghosts_zero = false;
+    return CPIC_OK;
+  });
 }
 
 int cpic_download_accumulators(cpic_context* c, void* out) {
-  if (int s = check_ctx(c)) return s;
-  if (!out) return fail(CPIC_ERR_INVALID, "null accumulators");
-  const size_t rs = c->rsz();
-  std::vector<unsigned char> buf(rs * 12 * c->ncells);
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  HIP_TRY(hipMemcpy(buf.data(), c->acc, buf.size(), hipMemcpyDeviceToHost));
-  unsigned char* dst = (unsigned char*)out;
-  for (long i = 0; i < c->ncells; ++i)
-    for (int k = 0; k < 12; ++k)
-      memcpy(&dst[((size_t)i * 12 + k) * rs], &buf[((size_t)k * c->ncells + i) * rs], rs);
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!out) return fail(CPIC_ERR_INVALID, "null accumulators");
+    const size_t rs = c->rsz();
+    std::vector<unsigned char> buf(rs * 12 * c->ncells);
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    HIP_TRY(hipMemcpy(buf.data(), c->acc, buf.size(), hipMemcpyDeviceToHost));
+    unsigned char* dst = (unsigned char*)out;
+    for (long i = 0; i < c->ncells; ++i)
+      for (int k = 0; k < 12; ++k)
+        memcpy(&dst[((size_t)i * 12 + k) * rs], &buf[((size_t)k * c->ncells + i) * rs], rs);
+    return CPIC_OK;
+  });
 }
 
 int cpic_load_interpolator_array(cpic_context* c) {
-  if (int s = check_ctx(c)) return s;
-  return with_real(c, [&](auto r) { return stage_load_interpolator<decltype(r)>(c); });
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    return with_real(c, [&](auto r) { return stage_load_interpolator<decltype(r)>(c); });
+  });
 }
 
 int cpic_clear_accumulator_array(cpic_context* c) {
-  if (int s = check_ctx(c)) return s;
-  return with_real(c, [&](auto r) { return stage_clear<decltype(r)>(c); });
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    return with_real(c, [&](auto r) { return stage_clear<decltype(r)>(c); });
+  });
 }
 
 int cpic_push(cpic_context* c, double qdt_2mc, double cdt_dx, double cdt_dy, double cdt_dz,
               double qsp, int32_t boundary) {
-  if (int s = check_ctx(c)) return s;
-  if (int s = check_periodic(boundary)) return s;
-  return with_real(c, [&](auto r) {
-    using R = decltype(r);
-    if (int t = check_headroom(c)) return t;
-    // the first push after an upload and then every kCompactInterval-th one
-    // pack the columns
-    const int compact = (c->pushes++ % kCompactInterval == 0) ? 1 : 0;
-    if (int t = stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1, compact))
-      return t;
-    return note_occupancy(c);
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (int s = check_periodic(boundary)) return s;
+    return with_real(c, [&](auto r) {
+      using R = decltype(r);
+      if (int t = check_headroom(c)) return t;
+      // the first push after an upload and then every kCompactInterval-th one
+      // pack the columns
+      const int compact = (c->pushes++ % kCompactInterval == 0) ? 1 : 0;
+      if (int t = stage_push<R>(c, (R)qdt_2mc, (R)cdt_dx, (R)cdt_dy, (R)cdt_dz, (R)qsp, 1, compact))
+        return t;
+      return note_occupancy(c);
+    });
   });
 }
 
 int cpic_uncenter_particles(cpic_context* c, double qdt_2mc) {
-  if (int s = check_ctx(c)) return s;
-  if (c->np == 0 || c->D == 0) return CPIC_OK;
-  return with_real(c, [&](auto r) {
-    using R = decltype(r);
-    k_uncenter_columns<R><<<cdiv(c->nc, kBlock), kBlock, 0, c->stream>>>(
-        (R*)c->cols[c->cur], c->cnt[c->cur], c->mask[c->cur], (const R*)c->interp, c->cg, c->D,
-        (int)c->nc_pad, (R)qdt_2mc);
-    CHECK_LAUNCH();
-    return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (c->np == 0 || c->D == 0) return CPIC_OK;
+    return with_real(c, [&](auto r) {
+      using R = decltype(r);
+      k_uncenter_columns<R><<<cdiv(c->nc, kBlock), kBlock, 0, c->stream>>>(
+          (R*)c->cols[c->cur], c->cnt[c->cur], c->mask[c->cur], (const R*)c->interp, c->cg, c->D,
+          (int)c->nc_pad, (R)qdt_2mc);
+      CHECK_LAUNCH();
+      return CPIC_OK;
+    });
   });
 }
 
 int cpic_unload_accumulator_array(cpic_context* c, double dx, double dy, double dz, double dt) {
-  if (int s = check_ctx(c)) return s;
-  return with_real(c, [&](auto r) {
-    using R = decltype(r);
-    return stage_unload<R>(c, (R)dx, (R)dy, (R)dz, (R)dt);
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    return with_real(c, [&](auto r) {
+      using R = decltype(r);
+      return stage_unload<R>(c, (R)dx, (R)dy, (R)dz, (R)dt);
+    });
   });
 }
 
 int cpic_advance_b(cpic_context* c, double px, double py, double pz) {
-  if (int s = check_ctx(c)) return s;
-  return with_real(c, [&](auto r) {
-    using R = decltype(r);
-    return stage_advance_b<R>(c, (R)px, (R)py, (R)pz);
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    return with_real(c, [&](auto r) {
+      using R = decltype(r);
+      return stage_advance_b<R>(c, (R)px, (R)py, (R)pz);
+    });
   });
 }
 
 int cpic_advance_e(cpic_context* c, double px, double py, double pz, double dt_eps0) {
-  if (int s = check_ctx(c)) return s;
-  return with_real(c, [&](auto r) {
-    using R = decltype(r);
-    return stage_advance_e<R>(c, (R)px, (R)py, (R)pz, (R)dt_eps0);
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    return with_real(c, [&](auto r) {
+      using R = decltype(r);
+      return stage_advance_e<R>(c, (R)px, (R)py, (R)pz, (R)dt_eps0);
+    });
   });
 }
 
 int cpic_set_field_solver(cpic_context* c, int32_t solver) {
-  if (int s = check_ctx(c)) return s;
-  if (solver != CPIC_FIELD_SOLVER_EM && solver != CPIC_FIELD_SOLVER_ES_1D)
-    return fail(CPIC_ERR_INVALID, "unknown field solver %d", solver);
-  // example/example.cpp:69-74
-  if (solver == CPIC_FIELD_SOLVER_ES_1D && (c->ny > 1 || c->nz > 1))
-    return fail(CPIC_ERR_UNSUPPORTED, "the ES field solver supports 1D (ny = nz = 1) only");
-  c->solver = solver;
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (solver != CPIC_FIELD_SOLVER_EM && solver != CPIC_FIELD_SOLVER_ES_1D)
+      return fail(CPIC_ERR_INVALID, "unknown field solver %d", solver);
+    // example/example.cpp:69-74
+    if (solver == CPIC_FIELD_SOLVER_ES_1D && (c->ny > 1 || c->nz > 1))
+      return fail(CPIC_ERR_UNSUPPORTED, "the ES field solver supports 1D (ny = nz = 1) only");
+    c->solver = solver;
+    return CPIC_OK;
+  });
 }
 
 int cpic_update_ghosts_j(cpic_context* c) {
-  if (int s = check_ctx(c)) return s;
-  return with_real(c, [&](auto r) { return stage_ghosts_j<decltype(r)>(c); });
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    return with_real(c, [&](auto r) { return stage_ghosts_j<decltype(r)>(c); });
+  });
 }
 
 int cpic_energies(cpic_context* c, double* e, double* b) {
-  if (int s = check_ctx(c)) return s;
-  int s = with_real(c, [&](auto r) { return launch_field_energy<decltype(r)>(c, c->red_out); });
-  if (s) return s;
-  double h[2];
-  HIP_TRY(hipMemcpyAsync(h, c->red_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (e) *e = h[0];
-  if (b) *b = h[1];
-  return collect_timing(c);
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    int s = with_real(c, [&](auto r) { return launch_field_energy<decltype(r)>(c, c->red_out); });
+    if (s) return s;
+    double h[2];
+    HIP_TRY(hipMemcpyAsync(h, c->red_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (e) *e = h[0];
+    if (b) *b = h[1];
+    return collect_timing(c);
+  });
 }
 
 int cpic_kinetic_energy(cpic_context* c, double me, double cc, double* ke) {
-  if (int s = check_ctx(c)) return s;
-  int s = with_real(c, [&](auto r) {
-    return launch_kinetic<decltype(r)>(c, me * cc * cc, c->red_out + 2);
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    int s = with_real(c, [&](auto r) {
+      return launch_kinetic<decltype(r)>(c, me * cc * cc, c->red_out + 2);
+    });
+    if (s) return s;
+    double h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, c->red_out + 2, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (ke) *ke = h;
+    return collect_timing(c);
   });
-  if (s) return s;
-  double h = 0;
-  HIP_TRY(hipMemcpyAsync(&h, c->red_out + 2, sizeof h, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (ke) *ke = h;
-  return collect_timing(c);
 }
 
 int cpic_gauss_residual(cpic_context* c, double dx, double dy, double dz, double eps0,
                         double qsp, int32_t mode, double* out, double* residual) {
-  if (int s = check_ctx(c)) return s;
-  if (mode != CPIC_GAUSS_BASELINE && mode != CPIC_GAUSS_CHANGE)
-    return fail(CPIC_ERR_INVALID, "Gauss check mode %d", (int)mode);
-  if (!(dx > 0 && dy > 0 && dz > 0 && eps0 > 0))
-    return fail(CPIC_ERR_INVALID, "cell sizes and eps0 must be positive");
-  if (mode == CPIC_GAUSS_CHANGE && !c->gauss_have_base)
-    return fail(CPIC_ERR_INVALID, "no Gauss baseline: call with CPIC_GAUSS_BASELINE first");
-  const size_t nc = (size_t)c->nc;
-  if (!c->gauss_q) {
-    if (int s = alloc((void**)&c->gauss_q, sizeof(double) * nc)) return s;
-    if (int s = alloc((void**)&c->gauss_base, sizeof(double) * nc)) return s;
-    if (int s = alloc((void**)&c->gauss_out, sizeof(unsigned long long) * 2)) return s;
-  }
-  int s = with_real(c, [&](auto r) {
-    using R = decltype(r);
-    if (int t = fill_zero<double>(c, c->gauss_q, nc)) return t;
-    if (int t = fill_zero<unsigned long long>(c, c->gauss_out, 2)) return t;
-    const int blocks = std::max(1, std::min(cdiv((long)nc, kBlock), 8192));
-    if (c->D > 0 && c->np > 0) {
-      const ColState<R> cs = col_state<R>(c, c->cur);
-      k_node_charge_columns<R><<<blocks, kBlock, 0, c->stream>>>(
-          cs, (int)c->nx, (int)c->ny, (int)c->nz, qsp, c->gauss_q);
-      CHECK_LAUNCH();
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (mode != CPIC_GAUSS_BASELINE && mode != CPIC_GAUSS_CHANGE)
+      return fail(CPIC_ERR_INVALID, "Gauss check mode %d", (int)mode);
+    if (!(dx > 0 && dy > 0 && dz > 0 && eps0 > 0))
+      return fail(CPIC_ERR_INVALID, "cell sizes and eps0 must be positive");
+    if (mode == CPIC_GAUSS_CHANGE && !c->gauss_have_base)
+      return fail(CPIC_ERR_INVALID, "no Gauss baseline: call with CPIC_GAUSS_BASELINE first");
+    const size_t nc = (size_t)c->nc;
+    if (!c->gauss_q) {
+      if (int s = alloc((void**)&c->gauss_q, sizeof(double) * nc)) return s;
+      if (int s = alloc((void**)&c->gauss_base, sizeof(double) * nc)) return s;
+      if (int s = alloc((void**)&c->gauss_out, sizeof(unsigned long long) * 2)) return s;
     }
-    // the residual overwrites the node charge in place (same index)
-    k_gauss_residual<R><<<blocks, kBlock, 0, c->stream>>>(
-        (const R*)c->fields, c->grid, c->gauss_q, 1.0 / dx, 1.0 / dy, 1.0 / dz,
-        1.0 / (dx * dy * dz * eps0), c->gauss_q, c->gauss_base,
-        mode == CPIC_GAUSS_BASELINE ? 0 : 1, c->gauss_out);
-    CHECK_LAUNCH();
-    return CPIC_OK;
+    int s = with_real(c, [&](auto r) {
+      using R = decltype(r);
+      if (int t = fill_zero<double>(c, c->gauss_q, nc)) return t;
+      if (int t = fill_zero<unsigned long long>(c, c->gauss_out, 2)) return t;
+      const int blocks = std::max(1, std::min(cdiv((long)nc, kBlock), 8192));
+      if (c->D > 0 && c->np > 0) {
+        const ColState<R> cs = col_state<R>(c, c->cur);
+        k_node_charge_columns<R><<<blocks, kBlock, 0, c->stream>>>(
+            cs, (int)c->nx, (int)c->ny, (int)c->nz, qsp, c->gauss_q);
+        CHECK_LAUNCH();
+      }
+      // the residual overwrites the node charge in place (same index)
+      k_gauss_residual<R><<<blocks, kBlock, 0, c->stream>>>(
+          (const R*)c->fields, c->grid, c->gauss_q, 1.0 / dx, 1.0 / dy, 1.0 / dz,
+          1.0 / (dx * dy * dz * eps0), c->gauss_q, c->gauss_base,
+          mode == CPIC_GAUSS_BASELINE ? 0 : 1, c->gauss_out);
+      CHECK_LAUNCH();
+      return CPIC_OK;
+    });
+    if (s) return s;
+    unsigned long long h[2];
+    HIP_TRY(hipMemcpyAsync(h, c->gauss_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    if (residual)
+      HIP_TRY(hipMemcpyAsync(residual, c->gauss_q, sizeof(double) * nc, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (mode == CPIC_GAUSS_BASELINE) c->gauss_have_base = true;
+    if (out) {
+      std::memcpy(&out[0], &h[0], sizeof(double));
+      std::memcpy(&out[1], &h[1], sizeof(double));
+    }
+    return collect_timing(c);
   });
-  if (s) return s;
-  unsigned long long h[2];
-  HIP_TRY(hipMemcpyAsync(h, c->gauss_out, sizeof h, hipMemcpyDeviceToHost, c->stream));
-  if (residual)
-    HIP_TRY(hipMemcpyAsync(residual, c->gauss_q, sizeof(double) * nc, hipMemcpyDeviceToHost,
-                           c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (mode == CPIC_GAUSS_BASELINE) c->gauss_have_base = true;
-  if (out) {
-    std::memcpy(&out[0], &h[0], sizeof(double));
-    std::memcpy(&out[1], &h[1], sizeof(double));
-  }
-  return collect_timing(c);
 }
 
 int cpic_sort_particles(cpic_context* c) {
-  // The particles are binned by cell at upload and stay binned: every push
-  // writes each particle into the column of the cell it ends in.
-  return check_ctx(c);
+  return guarded([&]() -> int {
+    // The particles are binned by cell at upload and stay binned: every push
+    // writes each particle into the column of the cell it ends in.
+    return check_ctx(c);
+  });
 }
 
 int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, double* hist) {
-  if (int s = check_ctx(c)) return s;
-  if (!sp) return fail(CPIC_ERR_INVALID, "null step parameters");
-  if (nsteps < 0) return fail(CPIC_ERR_INVALID, "negative step count");
-  if (int s = check_periodic(sp->boundary)) return s;
-  double* dhist = nullptr;
-  const int ei = sp->energy_interval;
-  const int ki = sp->kinetic_interval;
-  if (hist && (ei > 0 || ki > 0) && nsteps > 0) {
-    if (c->hist_cap < nsteps) {
-      HIP_TRY(hipStreamSynchronize(c->stream));
-      if (c->hist_dev) HIP_TRY(hipFree(c->hist_dev));
-      c->hist_dev = nullptr;
-      c->hist_cap = 0;
-      HIP_TRY(hipMalloc(&c->hist_dev, sizeof(double) * 3 * nsteps));
-      c->hist_cap = nsteps;
-    }
-    dhist = c->hist_dev;
-  }
-  int s = with_real(c, [&](auto r) {
-    using R = decltype(r);
-    const R px = (R)sp->px, py = (R)sp->py, pz = (R)sp->pz;
-    const R half = (R)0.5;
-    const R hpx = half * px, hpy = half * py, hpz = half * pz;
-    for (int64_t step = 1; step <= nsteps; ++step) {
-      int t;
-      if ((t = stage_load_interpolator<R>(c))) return t;
-      // the push overwrites every interior accumulator, so the clear is only
-      // needed while ghost accumulators may hold data
-      if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
-      if ((t = check_headroom(c))) return t;
-      const int ci = sp->compact_interval > 0 ? sp->compact_interval : kCompactInterval;
-      const int compact = (c->pushes++ % ci == 0) ? 1 : 0;
-      if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
-                             (R)sp->qsp, 0, compact)))
-        return t;
-      if ((t = note_occupancy(c))) return t;
-      if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;
-      if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
-      if ((t = stage_advance_e<R>(c, px, py, pz, (R)sp->dt_eps0))) return t;
-      if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
-      const bool do_e = ei > 0 && step % ei == 0, do_k = ki > 0 && step % ki == 0;
-      if (do_e || do_k) {
-        StageTimer tm(c, CPIC_STAGE_ENERGY);
-        double* row = dhist ? dhist + 3 * (step - 1) : c->red_out;
-        if (do_e && (t = launch_field_energy<R>(c, row))) return t;
-        if (do_k && (t = launch_kinetic<R>(c, (double)sp->me * sp->c * sp->c, row + 2))) return t;
-        if ((t = tm.finish())) return t;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!sp) return fail(CPIC_ERR_INVALID, "null step parameters");
+    if (nsteps < 0) return fail(CPIC_ERR_INVALID, "negative step count");
+    if (int s = check_periodic(sp->boundary)) return s;
+    double* dhist = nullptr;
+    const int ei = sp->energy_interval;
+    const int ki = sp->kinetic_interval;
+    if (hist && (ei > 0 || ki > 0) && nsteps > 0) {
+      if (c->hist_cap < nsteps) {
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        if (c->hist_dev) HIP_TRY(hipFree(c->hist_dev));
+        c->hist_dev = nullptr;
+        c->hist_cap = 0;
+        HIP_TRY(hipMalloc(&c->hist_dev, sizeof(double) * 3 * nsteps));
+        c->hist_cap = nsteps;
       }
+      dhist = c->hist_dev;
     }
-    return CPIC_OK;
-  });
-  if (s == CPIC_OK && dhist) {
-    std::vector<double> h(3 * nsteps);
-    hipError_t e = hipMemcpyAsync(h.data(), dhist, sizeof(double) * 3 * nsteps,
-                                  hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    if (e != hipSuccess) {
-      s = fail(CPIC_ERR_HIP, "history download failed: %s", hipGetErrorString(e));
-    } else {
+    int s = with_real(c, [&](auto r) {
+      using R = decltype(r);
+      const R px = (R)sp->px, py = (R)sp->py, pz = (R)sp->pz;
+      const R half = (R)0.5;
+      const R hpx = half * px, hpy = half * py, hpz = half * pz;
       for (int64_t step = 1; step <= nsteps; ++step) {
-        if (ei > 0 && step % ei == 0)
-          for (int k = 0; k < 2; ++k) hist[3 * (step - 1) + k] = h[3 * (step - 1) + k];
-        if (ki > 0 && step % ki == 0) hist[3 * (step - 1) + 2] = h[3 * (step - 1) + 2];
+        int t;
+        if ((t = stage_load_interpolator<R>(c))) return t;
+        // the push overwrites every interior accumulator, so the clear is only
+        // needed while ghost accumulators may hold data
+        if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
+        if ((t = check_headroom(c))) return t;
+        const int ci = sp->compact_interval > 0 ? sp->compact_interval : kCompactInterval;
+        const int compact = (c->pushes++ % ci == 0) ? 1 : 0;
+        if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
+                               (R)sp->qsp, 0, compact)))
+          return t;
+        if ((t = note_occupancy(c))) return t;
+        if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;
+        if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
+        if ((t = stage_advance_e<R>(c, px, py, pz, (R)sp->dt_eps0))) return t;
+        if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
+        const bool do_e = ei > 0 && step % ei == 0, do_k = ki > 0 && step % ki == 0;
+        if (do_e || do_k) {
+          StageTimer tm(c, CPIC_STAGE_ENERGY);
+          double* row = dhist ? dhist + 3 * (step - 1) : c->red_out;
+          if (do_e && (t = launch_field_energy<R>(c, row))) return t;
+          if (do_k && (t = launch_kinetic<R>(c, (double)sp->me * sp->c * sp->c, row + 2))) return t;
+          if ((t = tm.finish())) return t;
+        }
+      }
+      return CPIC_OK;
+    });
+    if (s == CPIC_OK && dhist) {
+      std::vector<double> h(3 * nsteps);
+      hipError_t e = hipMemcpyAsync(h.data(), dhist, sizeof(double) * 3 * nsteps,
+                                    hipMemcpyDeviceToHost, c->stream);
+      if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+      if (e != hipSuccess) {
+        s = fail(CPIC_ERR_HIP, "history download failed: %s", hipGetErrorString(e));
+      } else {
+        for (int64_t step = 1; step <= nsteps; ++step) {
+          if (ei > 0 && step % ei == 0)
+            for (int k = 0; k < 2; ++k) hist[3 * (step - 1) + k] = h[3 * (step - 1) + k];
+          if (ki > 0 && step % ki == 0) hist[3 * (step - 1) + 2] = h[3 * (step - 1) + 2];
+        }
       }
     }
-  }
-  if (s) return s;
-  if ((s = check_device_errors(c))) return s;
-  return collect_timing(c);
+    if (s) return s;
+    if ((s = check_device_errors(c))) return s;
+    return collect_timing(c);
+  });
 }
 
 int cpic_set_timing(cpic_context* c, int32_t enabled) {
-  return cpic_set_timing_stages(c, enabled ? (1u << CPIC_NUM_STAGES) - 1 : 0u);
+  return guarded([&]() -> int {
+    return cpic_set_timing_stages(c, enabled ? (1u << CPIC_NUM_STAGES) - 1 : 0u);
+  });
 }
 
 int cpic_set_timing_stages(cpic_context* c, uint32_t stages) {
-  if (int s = check_ctx(c)) return s;
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (int s = collect_timing(c)) return s;
-  c->timing = stages & ((1u << CPIC_NUM_STAGES) - 1);
-  for (int k = 0; k < CPIC_NUM_STAGES; ++k) {
-    c->stage_count[k] = 0;
-    c->stage_ms[k] = 0;
-  }
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int s = collect_timing(c)) return s;
+    c->timing = stages & ((1u << CPIC_NUM_STAGES) - 1);
+    for (int k = 0; k < CPIC_NUM_STAGES; ++k) {
+      c->stage_count[k] = 0;
+      c->stage_ms[k] = 0;
+    }
+    return CPIC_OK;
+  });
 }
 
 int cpic_stage_times(cpic_context* c, int64_t* counts, double* ms) {
-  if (int s = check_ctx(c)) return s;
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (int s = collect_timing(c)) return s;
-  for (int k = 0; k < CPIC_NUM_STAGES; ++k) {
-    if (counts) counts[k] = c->stage_count[k];
-    if (ms) ms[k] = c->stage_ms[k];
-  }
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (int s = collect_timing(c)) return s;
+    for (int k = 0; k < CPIC_NUM_STAGES; ++k) {
+      if (counts) counts[k] = c->stage_count[k];
+      if (ms) ms[k] = c->stage_ms[k];
+    }
+    return CPIC_OK;
+  });
 }
 
 int cpic_last_push_movers(cpic_context* c, int64_t* movers) {
-  if (int s = check_ctx(c)) return s;
-  unsigned int m = 0;
-  HIP_TRY(hipMemcpyAsync(&m, c->movers, sizeof m, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (movers) *movers = m;
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    unsigned int m = 0;
+    HIP_TRY(hipMemcpyAsync(&m, c->movers, sizeof m, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (movers) *movers = m;
+    return CPIC_OK;
+  });
 }
 
 int cpic_device_errors(cpic_context* c, int32_t* flags) {
-  if (int s = check_ctx(c)) return s;
-  int f = 0;
-  HIP_TRY(hipMemcpyAsync(&f, c->err, sizeof f, hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(hipStreamSynchronize(c->stream));
-  if (flags) *flags = f;
-  return CPIC_OK;
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    int f = 0;
+    HIP_TRY(hipMemcpyAsync(&f, c->err, sizeof f, hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (flags) *flags = f;
+    return CPIC_OK;
+  });
 }
 
 }  // extern "C"

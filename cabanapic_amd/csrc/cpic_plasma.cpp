@@ -183,11 +183,18 @@ void fill_parallel(const cpic_plasma_spec& s, int64_t n, void* x, void* y, void*
   if (nt > 16) nt = 16;
   if (n < 100000) nt = 1;
   std::vector<std::thread> pool;
-  for (unsigned t = 0; t < nt; ++t) {
-    const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
-    pool.emplace_back([&, lo, hi] {
-      fill<R>(s, cp, lo, hi, (R*)x, (R*)y, (R*)z, (R*)ux, (R*)uy, (R*)uz, (R*)w, cell);
-    });
+  pool.reserve(nt);
+  try {
+    for (unsigned t = 0; t < nt; ++t) {
+      const int64_t lo = n * t / nt, hi = n * (t + 1) / nt;
+      pool.emplace_back([&, lo, hi] {
+        fill<R>(s, cp, lo, hi, (R*)x, (R*)y, (R*)z, (R*)ux, (R*)uy, (R*)uz, (R*)w, cell);
+      });
+    }
+  } catch (...) {
+    // a thread that could not start: finish the started ones, then report
+    for (auto& th : pool) th.join();
+    throw;
   }
   for (auto& th : pool) th.join();
 }
@@ -223,10 +230,15 @@ extern "C" int cpic_init_plasma(const cpic_plasma_spec* s, int32_t real_size, in
   if (n != s->nx * s->ny * s->nz * s->nppc) return CPIC_ERR_INVALID;
   if (s->profile != 0 && s->profile != 1) return CPIC_ERR_INVALID;
   if (s->profile == 1 && !column_spec_ok(*s)) return CPIC_ERR_INVALID;
-  if (real_size == 4)
-    fill_parallel<float>(*s, n, x, y, z, ux, uy, uz, w, cell);
-  else
-    fill_parallel<double>(*s, n, x, y, z, ux, uy, uz, w, cell);
+  // host allocation and thread start-up failures are returned, not thrown
+  try {
+    if (real_size == 4)
+      fill_parallel<float>(*s, n, x, y, z, ux, uy, uz, w, cell);
+    else
+      fill_parallel<double>(*s, n, x, y, z, ux, uy, uz, w, cell);
+  } catch (...) {
+    return CPIC_ERR_INVALID;
+  }
   return CPIC_OK;
 }
 
@@ -236,9 +248,13 @@ extern "C" int cpic_init_column_fields(const cpic_plasma_spec* s, int32_t real_s
   if (real_size != 4 && real_size != 8) return CPIC_ERR_INVALID;
   if (s->nx < 1 || s->ny < 1 || s->nz < 1 || s->profile != 1) return CPIC_ERR_INVALID;
   if (!column_spec_ok(*s)) return CPIC_ERR_INVALID;
-  if (real_size == 4)
-    column_fields<float>(*s, (float*)fields);
-  else
-    column_fields<double>(*s, (double*)fields);
+  try {
+    if (real_size == 4)
+      column_fields<float>(*s, (float*)fields);
+    else
+      column_fields<double>(*s, (double*)fields);
+  } catch (...) {
+    return CPIC_ERR_INVALID;
+  }
   return CPIC_OK;
 }

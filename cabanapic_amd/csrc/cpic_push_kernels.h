@@ -5,7 +5,7 @@
 // cells form a group, and row r of a group is 64 consecutive slots, one per
 // cell:
 //     slot(ci, r) = ((ci / 64) * D + r) * 64 + ci % 64
-// A slot holds one particle as 8 reals (x, y, z, ux, uy, uz, w, unused), two
+// A slot holds one particle as 8 reals (x, y, z, ux, uy, uz, w, tag), two
 // 4-wide vector accesses, so a scattered particle store touches one 32-byte
 // (float) or 64-byte (double) piece of memory.
 // A wave owns the 64 cells of one group, one cell per lane.  Looping over the
@@ -52,12 +52,12 @@
 
 namespace cpic {
 
-// A crossing record: pos[3], rem[3], u[3], w, then padding to whole 32-byte
+// A crossing record: pos[3], rem[3], u[3], w, tag, then padding to whole 32-byte
 // sectors (64 bytes in float, 96 in double), so that a record write never
 // leaves a partially written sector, which memory would have to read back.
 template <class R> struct RecordLen { static constexpr int value = sizeof(R) == 4 ? 16 : 12; };
 constexpr int record_bytes(int real_size) { return real_size == 4 ? 64 : 96; }
-constexpr int kSlotLen = 8;     // x, y, z, ux, uy, uz, w, pad
+constexpr int kSlotLen = 8;     // x, y, z, ux, uy, uz, w, tag
 
 __device__ __forceinline__ size_t col_slot(int ci, int row, int D) {
   return ((size_t)(ci >> 6) * (size_t)D + (size_t)row) * 64 + (size_t)(ci & 63);
@@ -378,12 +378,13 @@ template <class R> struct Overflow {
 template <class R>
 __device__ __forceinline__ void put_record(const RecordList<R>& L, const Overflow<R>& ovf,
                                            const CellGrid& g, int ci, int d, int& k, const R pos[3],
-                                           const R rem[3], R ux, R uy, R uz, R w, int* err) {
+                                           const R rem[3], R ux, R uy, R uz, R w, R tag,
+                                           int* err) {
   typedef typename Vec4T<R>::type V4;
   V4 a, b, c;
   a.x = pos[0]; a.y = pos[1]; a.z = pos[2]; a.w = rem[0];
   b.x = rem[1]; b.y = rem[2]; b.z = ux;     b.w = uy;
-  c.x = uz;     c.y = w;      c.z = 0;      c.w = 0;
+  c.x = uz;     c.y = w;      c.z = tag;    c.w = 0;
   V4* dst;
   if (k < L.M) {
     dst = reinterpret_cast<V4*>(record_at(L, ci, d, k));
@@ -406,11 +407,11 @@ __device__ __forceinline__ void put_record(const RecordList<R>& L, const Overflo
 
 template <class R>
 __device__ __forceinline__ void store_particle(const ColState<R>& cs, int ci, int row, R x, R y, R z,
-                                               R ux, R uy, R uz, R w) {
+                                               R ux, R uy, R uz, R w, R tag) {
   typedef typename Vec4T<R>::type V4;
   V4 a, b;
   a.x = x;  a.y = y;  a.z = z;  a.w = ux;
-  b.x = uy; b.y = uz; b.z = w;  b.w = 0;
+  b.x = uy; b.y = uz; b.z = w;  b.w = tag;
   V4* dst = reinterpret_cast<V4*>(cs.out + col_slot(ci, row, cs.D) * kSlotLen);
   dst[0] = a;
   dst[1] = b;
@@ -453,9 +454,10 @@ template <class R> struct PushPass {
 // the departure mask of face d.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
+  if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // an earlier push failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
   const bool valid = ci < a.g.nc;
-  const int len = valid ? a.cs.cnt_in[ci] : 0;
+  const int len = valid ? min(a.cs.cnt_in[ci], a.cs.D) : 0;
   const int lmax = wave_max_int(len);
   const int v = valid ? interior_to_voxel(a.g, ci) : 0;
   typedef typename Vec4T<R>::type V4;
@@ -539,7 +541,7 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
         dw4 |= d == 4 ? bit : 0ull;
         dw5 |= d == 5 ? bit : 0ull;
       }
-      store_particle(a.cs, ci, row, px, py, pz, ux, uy, uz, w);
+      store_particle(a.cs, ci, row, px, py, pz, ux, uy, uz, w, pb.w);
     }
   };
 
@@ -626,6 +628,7 @@ __device__ __forceinline__ int sel6(int d, int a0, int a1, int a2, int a3, int a
 // order) in one loop, so a wave iterates max over lanes of the lane totals.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
+  if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // this push already failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = ci < a.g.nc;
   const size_t NC = a.g.nc;
@@ -653,7 +656,7 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
     const int v = valid ? interior_to_voxel(a.g, ci) : 0;
     RowCursor<R> rows;
     {
-      const int l = valid ? a.cs.cnt_out[ci] : 0;
+      const int l = valid ? min(a.cs.cnt_out[ci], a.cs.D) : 0;
       unsigned long long w0 = valid ? a.cs.mask_out[ci] : 0ull;
       asm volatile("" : "+v"(w0));  // wait for the word here, before the loop
       rows.start(l, w0);
@@ -685,18 +688,18 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
     auto process = [&](const V4& r0, const V4& r1, const V4& r2) {
       R pos[3] = {r0.x, r0.y, r0.z};
       R rem[3] = {r0.w, r1.x, r1.y};
-      const R ux = r1.z, uy = r1.w, uz = r2.x, w = r2.y;
+      const R ux = r1.z, uy = r1.w, uz = r2.x, w = r2.y, tag = r2.z;
       const R q = w * a.qsp;
       const int d2 = streak(pos, rem, q, acc);
       if (d2 < 0) {
         const int row = rows.next(a.cs, ci);
         if (row >= 0)
-          store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w);
+          store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w, tag);
         else
           atomicOr(a.err, kErrColumnFull);
       } else {
         int kk = a.last ? a.out.M : kd[d2];  // last pass: straight to overflow
-        put_record(a.out, a.ovf, a.g, ci, d2, kk, pos, rem, ux, uy, uz, w, a.err);
+        put_record(a.out, a.ovf, a.g, ci, d2, kk, pos, rem, ux, uy, uz, w, tag, a.err);
         if (!a.last) kd[d2] = kk;
       }
     };
@@ -741,9 +744,10 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
 // for those stores.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
+  if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // an earlier push failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
   const bool valid = ci < a.g.nc;
-  const int len = valid ? a.cs.cnt_in[ci] : 0;
+  const int len = valid ? min(a.cs.cnt_in[ci], a.cs.D) : 0;
   // rows walked: the longest column of the wave, rounded up to even so that
   // both rows of every loop iteration are processed (D is a multiple of 8,
   // so the extra row is a row past every column's end)
@@ -844,7 +848,7 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
       m_out |= 1ull << b;
       hi = j + 1;
       o0.x = px; o0.y = py; o0.z = pz; o0.w = ux;
-      o1.x = uy; o1.y = uz; o1.z = w;  o1.w = 0;
+      o1.x = uy; o1.y = uz; o1.z = w;  o1.w = pb.w;
     }
     const unsigned long long occ = __ballot(have);
     const bool st = (occ >> line0) & ((1ull << kLine) - 1);
@@ -929,6 +933,7 @@ template <class R> struct ArrivePass {
 // streak.  Afterwards each cell releases the rows of its own departures.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
+  if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // this push already failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
   const bool valid = ci < a.g.nc;
   const size_t MP = a.cs.nc_pad;
@@ -949,7 +954,7 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
   const int v = valid ? interior_to_voxel(a.g, ci) : 0;
   RowCursor<R> rows;
   {
-    const int l = valid ? a.cs.cnt_out[ci] : 0;
+    const int l = valid ? min(a.cs.cnt_out[ci], a.cs.D) : 0;
     unsigned long long w0 = valid ? a.cs.mask_out[ci] : 0ull;
     settle(w0);
     rows.start(l, w0);
@@ -1023,12 +1028,12 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
       if (d2n < 0) {
         const int row = rows.next(a.cs, ci);
         if (row >= 0)
-          store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w8);
+          store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w8, pb.w);
         else
           atomicOr(a.err, kErrColumnFull);
       } else {
         int kk = kd[d2n];
-        put_record(a.out, a.ovf, a.g, ci, d2n, kk, pos, rem, ux, uy, uz, w8, a.err);
+        put_record(a.out, a.ovf, a.g, ci, d2n, kk, pos, rem, ux, uy, uz, w8, pb.w, a.err);
         kd[d2n] = kk;
       }
     };
@@ -1084,6 +1089,7 @@ template <class R> struct SpillPass {
 
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_spill_pass(SpillPass<R> a) {
+  if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // this push already failed
   const unsigned total = min(*a.ovf.count, (unsigned)a.ovf.cap);
   typedef typename Vec4T<R>::type V4;
   for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
@@ -1091,7 +1097,7 @@ __global__ void __launch_bounds__(kBlock) k_spill_pass(SpillPass<R> a) {
     const V4 r0 = src[0], r1 = src[1], r2 = src[2];
     R pos[3] = {r0.x, r0.y, r0.z};
     R rem[3] = {r0.w, r1.x, r1.y};
-    const R ux = r1.z, uy = r1.w, uz = r2.x, w = r2.y;
+    const R ux = r1.z, uy = r1.w, uz = r2.x, w = r2.y, tag = r2.z;
     const R q = w * a.qsp;
     int ci = a.ovf.cell[i];
     if ((unsigned)ci >= (unsigned)a.g.nc) {
@@ -1118,11 +1124,14 @@ __global__ void __launch_bounds__(kBlock) k_spill_pass(SpillPass<R> a) {
       atomicOr(a.err, kErrStreakLimit);
       continue;
     }
+    // a row is taken only below D: a failed claim is handed back, so the
+    // count never exceeds the column's capacity
     const int row = atomicAdd(a.cs.cnt_out + ci, 1);
     if (row < a.cs.D) {
-      store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w);
+      store_particle(a.cs, ci, row, pos[0], pos[1], pos[2], ux, uy, uz, w, tag);
       atomicOr(a.cs.mask_out + (size_t)(row >> 6) * a.cs.nc_pad + ci, 1ull << (row & 63));
     } else {
+      atomicSub(a.cs.cnt_out + ci, 1);
       atomicOr(a.err, kErrColumnFull);
     }
   }
@@ -1148,7 +1157,7 @@ k_uncenter_columns(R* __restrict__ cols, const int* __restrict__ cnt,
   const R one = 1.;
   const R one_third = 1. / 3.;
   const R two_fifteenths = 2. / 15.;
-  const int n = cnt[ci];
+  const int n = min(cnt[ci], D);
   unsigned long long m = 0;
   for (int j = 0; j < n; ++j) {
     if ((j & 63) == 0) m = mask[(size_t)(j >> 6) * nc_pad + ci];
@@ -1189,7 +1198,7 @@ __global__ void __launch_bounds__(kBlock)
 k_kinetic_columns(ColState<R> cs, int nc, double* __restrict__ partials) {
   double v[1] = {0};
   for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nc; ci += gridDim.x * blockDim.x) {
-    const int n = cs.cnt_in[ci];
+    const int n = min(cs.cnt_in[ci], cs.D);
     unsigned long long m = 0;
     for (int j = 0; j < n; ++j) {
       if ((j & 63) == 0) m = cs.mask_in[(size_t)(j >> 6) * cs.nc_pad + ci];
@@ -1218,7 +1227,7 @@ k_node_charge_columns(ColState<R> cs, int nx, int ny, int nz, double qsp,
   const int nc = nx * ny * nz;
   for (int ci = blockIdx.x * blockDim.x + threadIdx.x; ci < nc; ci += gridDim.x * blockDim.x) {
     double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int n = cs.cnt_in[ci];
+    const int n = min(cs.cnt_in[ci], cs.D);
     unsigned long long m = 0;
     for (int j = 0; j < n; ++j) {
       if ((j & 63) == 0) m = cs.mask_in[(size_t)(j >> 6) * cs.nc_pad + ci];
@@ -1242,6 +1251,135 @@ k_node_charge_columns(ColState<R> cs, int nx, int ny, int nz, double qsp,
       atomicAdd(q + jx + nx * (jy + ny * jz), s[k]);
     }
   }
+}
+
+}  // namespace cpic
+
+namespace cpic {
+
+// ---- column maintenance (upload, download, regrowth) -----------------------
+// A slot's eighth real carries the particle's tag (its index in the upload,
+// as the bits of a 32-bit or 64-bit integer), copied unchanged by every pass.
+
+template <class R> struct TagBits;
+template <> struct TagBits<float> {
+  __device__ static float make(long long i) { return __int_as_float((int)i); }
+  __device__ static long long get(float t) { return (long long)__float_as_int(t); }
+};
+template <> struct TagBits<double> {
+  __device__ static double make(long long i) { return __longlong_as_double(i); }
+  __device__ static long long get(double t) { return __double_as_longlong(t); }
+};
+
+// Upload: particle p (SoA src, 7 arrays of stride n) goes to column slot
+// slot[p], which the host computed (cell and rank in the cell, input order).
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_bin_scatter(const R* __restrict__ src, const long long* __restrict__ slot, long n,
+              R* __restrict__ cols) {
+  typedef typename Vec4T<R>::type V4;
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long)gridDim.x * blockDim.x) {
+    V4 a, b;
+    a.x = src[p];
+    a.y = src[n + p];
+    a.z = src[2 * n + p];
+    a.w = src[3 * n + p];
+    b.x = src[4 * n + p];
+    b.y = src[5 * n + p];
+    b.z = src[6 * n + p];
+    b.w = TagBits<R>::make(p);
+    V4* dst = reinterpret_cast<V4*>(cols + (size_t)slot[p] * kSlotLen);
+    dst[0] = a;
+    dst[1] = b;
+  }
+}
+
+// Occupancy masks of packed columns: rows [0, cnt) of every cell.
+__global__ void __launch_bounds__(kBlock)
+k_masks_packed(const int* __restrict__ cnt, unsigned long long* __restrict__ mask, int nc,
+               int nc_pad, int MW) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= nc_pad) return;
+  const int n = ci < nc ? cnt[ci] : 0;
+  for (int w = 0; w < MW; ++w) mask[(size_t)w * nc_pad + ci] = low_bits(n - 64 * w);
+}
+
+// Particles held by each cell (occupied rows below its count).
+__global__ void __launch_bounds__(kBlock)
+k_column_occupancy(const int* __restrict__ cnt, const unsigned long long* __restrict__ mask,
+                   int nc, int nc_pad, int D, int* __restrict__ occ) {
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= nc) return;
+  const int n = min(cnt[ci], D);
+  int k = 0;
+  for (int w = 0; 64 * w < n; ++w)
+    k += __popcll(mask[(size_t)w * nc_pad + ci] & low_bits(n - 64 * w));
+  occ[ci] = k;
+}
+
+// Download: the particles of cell ci, occupied rows in row order, go to
+// positions first[ci], first[ci] + 1, ... of the SoA dst (7 arrays of stride
+// n), with their voxel in cell[] and their tag in tag[] (may be null).
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_gather_columns(const R* __restrict__ cols, const int* __restrict__ cnt,
+                 const unsigned long long* __restrict__ mask, CellGrid g, int D, int nc_pad,
+                 const long long* __restrict__ first, long n, R* __restrict__ dst,
+                 int* __restrict__ cell, long long* __restrict__ tag) {
+  typedef typename Vec4T<R>::type V4;
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= g.nc) return;
+  const int len = min(cnt[ci], D);
+  const int v = interior_to_voxel(g, ci);
+  long o = first[ci];
+  unsigned long long m = 0;
+  for (int j = 0; j < len; ++j) {
+    if ((j & 63) == 0) m = mask[(size_t)(j >> 6) * nc_pad + ci];
+    if (!((m >> (j & 63)) & 1ull)) continue;
+    if (o >= n) break;  // more particles than the host counted: caught by the host
+    const V4* s = reinterpret_cast<const V4*>(cols + col_slot(ci, j, D) * kSlotLen);
+    const V4 a = s[0], b = s[1];
+    dst[o] = a.x;
+    dst[n + o] = a.y;
+    dst[2 * n + o] = a.z;
+    dst[3 * n + o] = a.w;
+    dst[4 * n + o] = b.x;
+    dst[5 * n + o] = b.y;
+    dst[6 * n + o] = b.z;
+    cell[o] = v;
+    if (tag) tag[o] = TagBits<R>::get(b.w);
+    ++o;
+  }
+}
+
+// Regrowth: the occupied rows of every column, packed in row order, into
+// columns of D_out rows.
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_relayout(const R* __restrict__ in, const int* __restrict__ cnt_in,
+           const unsigned long long* __restrict__ mask_in, int D_in, R* __restrict__ out,
+           int* __restrict__ cnt_out, int D_out, int nc, int nc_pad, int* __restrict__ err) {
+  typedef typename Vec4T<R>::type V4;
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci >= nc_pad) return;
+  const int len = ci < nc ? min(cnt_in[ci], D_in) : 0;
+  int k = 0;
+  unsigned long long m = 0;
+  for (int j = 0; j < len; ++j) {
+    if ((j & 63) == 0) m = mask_in[(size_t)(j >> 6) * nc_pad + ci];
+    if (!((m >> (j & 63)) & 1ull)) continue;
+    if (k >= D_out) {
+      atomicOr(err, kErrColumnFull);
+      break;
+    }
+    const V4* s = reinterpret_cast<const V4*>(in + col_slot(ci, j, D_in) * kSlotLen);
+    V4* d = reinterpret_cast<V4*>(out + col_slot(ci, k, D_out) * kSlotLen);
+    d[0] = s[0];
+    d[1] = s[1];
+    ++k;
+  }
+  cnt_out[ci] = k;
 }
 
 }  // namespace cpic

@@ -125,6 +125,18 @@ class Simulation:
             self._h, n, *[_ptr(out[k]) for k in PARTICLE_MEMBERS], _ptr(out["cell"])))
         return out
 
+    def download_particle_tags(self):
+        """Upload index of every particle, in download_particles order."""
+        n = self.num_particles()
+        t = np.empty(n, np.int64)
+        check(self.lib.cpic_download_particle_tags(self._h, n, _ptr(t)))
+        return t
+
+    def column_rows(self):
+        r = ctypes.c_int64()
+        check(self.lib.cpic_column_rows(self._h, ctypes.byref(r)))
+        return r.value
+
     def upload_fields(self, fields):
         f = self._arr(fields)
         if f.size != 9 * self.ncells:

@@ -56,9 +56,10 @@ typedef struct cpic_config {
    * rounded up to a multiple of 64), or every particle of the context where that costs at most 64 MB of
    * columns, and at least column_rows.  Before each push the library
    * checks the previous push's fullest column; within max(8, rows/4) of
-   * the capacity it re-lays the columns out with 1.5x the rows (bunching
-   * physics: a nonlinear two-stream phase reaches 4x the mean count).  A
-   * push that still overflows a column fails with CPIC_ERR_DEVICE. */
+   * the capacity it re-lays the columns out on the device with 1.5x the
+   * rows (bunching physics: a nonlinear two-stream phase reaches 4x the
+   * mean count).  A push that still overflows a column fails with
+   * CPIC_ERR_DEVICE. */
   int64_t column_rows;
 } cpic_config;
 
@@ -94,7 +95,13 @@ int cpic_get_stream(cpic_context *ctx, void **stream);
 
 /* ---- data movement (host <-> HBM), synchronous -------------------------- */
 
-/* Particle cells must be interior voxels; anything else is rejected. */
+/* Particle cells must be interior voxels; anything else is rejected.  The
+ * particles are binned into the per-cell columns on the device (the host
+ * computes each particle's slot; the arrays pass once over PCIe).  A new
+ * column capacity is allocated before the old columns are released, so an
+ * allocation failure leaves the resident particles unchanged; a failure
+ * after that leaves the context without particles (cpic_num_particles 0).
+ * An upload is a fresh particle state: it clears the device error flags. */
 int cpic_upload_particles(cpic_context *ctx, int64_t n, const void *x,
                           const void *y, const void *z, const void *ux,
                           const void *uy, const void *uz, const void *w,
@@ -102,7 +109,13 @@ int cpic_upload_particles(cpic_context *ctx, int64_t n, const void *x,
 int cpic_download_particles(cpic_context *ctx, int64_t n, void *x, void *y,
                             void *z, void *ux, void *uy, void *uz, void *w,
                             int32_t *cell);
+/* The upload index of every particle, in the order cpic_download_particles
+ * returns them (no reference counterpart: the reference's particles carry no
+ * identity; the index lets a caller follow particles through the columns). */
+int cpic_download_particle_tags(cpic_context *ctx, int64_t n, int64_t *tags);
 int cpic_num_particles(cpic_context *ctx, int64_t *n);
+/* Rows of each cell's particle column (no reference counterpart). */
+int cpic_column_rows(cpic_context *ctx, int64_t *rows);
 int cpic_upload_fields(cpic_context *ctx, const void *fields);
 int cpic_download_fields(cpic_context *ctx, void *fields);
 int cpic_upload_interpolators(cpic_context *ctx, const void *interp);
@@ -237,7 +250,10 @@ int cpic_set_timing_stages(cpic_context *ctx, uint32_t stages);
 int cpic_stage_times(cpic_context *ctx, int64_t *counts, double *ms);
 
 /* Particles that took the move_p branch in the last push, and the device
- * error flags raised since creation (0 = none). */
+ * error flags raised since the last particle upload (0 = none).  A push
+ * that raises a flag leaves the particle state undefined: later pushes do
+ * nothing on the device and cpic_push / cpic_step return CPIC_ERR_DEVICE
+ * until particles are uploaded again. */
 int cpic_last_push_movers(cpic_context *ctx, int64_t *movers);
 int cpic_device_errors(cpic_context *ctx, int32_t *flags);
 

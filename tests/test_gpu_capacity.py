@@ -51,10 +51,18 @@ def test_columns_grow_under_bunching(dtype):
     counts0 = np.bincount(st.cell).max()
     with Simulation(NX, NY, NZ, st.np, dtype) as sim:
         sim.upload_particles(*[st.p[m] for m in states.MEMBERS], st.cell)
+        rows0 = sim.column_rows()
         sim.step(dict(k, energy_interval=0), STEPS)
         assert sim.device_errors() == 0
         got = sim.download_particles()
         f = sim.download_fields()
+        rows1 = sim.column_rows()
+        # a new upload sizes its columns for its own particles: the
+        # regrowth of the bunched run does not carry over
+        sim.upload_particles(*[st.p[m] for m in states.MEMBERS], st.cell)
+        assert sim.column_rows() == rows0
+    print(f"column rows {rows0} at upload, {rows1} after the run")
+    assert rows1 > rows0
     h, _ = st.run(states.consts_vector(k), STEPS)
     top = np.bincount(got["cell"]).max()
     print(f"{np.dtype(dtype).name}: largest cell count {counts0} at upload, {top} after "

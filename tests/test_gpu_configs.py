@@ -11,7 +11,8 @@ every 32nd) and steps after long runs, where the columns hold holes.
 
 Configurations (cells, particles per cell):
   1D two-stream      32 x 1 x 1, 2 x 32, 1000 steps, every step checked
-  3D two-stream      64^3, 64 (16,777,216 particles), 200 steps
+  3D two-stream      64^3, 64 (16,777,216 particles), 200 steps, every FP32
+                     push and the first 50 FP64 pushes checked
   Weibel             128^3, 32 (67,108,864 particles), the benchmark workload
   electron column    192 x 192 x 64, 32 (75,497,472 particles), guide field
 The 3D two-stream and Weibel runs also compare their FP32 and FP64 field
@@ -165,15 +166,54 @@ def _run_checked(deck, checks, total):
     return hist
 
 
-@pytest.mark.timeout(600)
-def test_two_stream_3d_full_size():
+def _assert_same_by_tag(got, tags, ref, ref_tags, what):
+    """got (tagged tags) against ref (tagged ref_tags): particle for particle,
+    matched by upload index, bit for bit."""
+    n = ref_tags.size
+    assert tags.size == n, f"{what}: particle count changed"
+    inv = np.empty(n, np.int64)
+    inv[ref_tags] = np.arange(n)
+    idx = inv[tags]
+    bad = []
+    for m in ("cell",) + MEMBERS:
+        a = np.ascontiguousarray(got[m])
+        b = np.ascontiguousarray(ref[m][idx])
+        if not np.array_equal(a.view(np.uint8), b.view(np.uint8)):
+            k = int(np.count_nonzero(_bits(a) != _bits(b)))
+            bad.append(f"{m} ({k} of {n})")
+    assert not bad, f"{what}: particles differ from the oracle's: {', '.join(bad)}"
+
+
+@pytest.mark.timeout(1200)
+def test_two_stream_3d_every_push():
     """64^3 cells x 64 ppc (16.8M particles), 200 steps in FP32 and FP64:
-    pushes bit-identical to the oracle at steps 1, 2, 33 (a packing step
-    after holes) and 200; FP32 field energy within 1e-3 of FP64."""
+    every one of the 200 FP32 pushes, and the first 50 FP64 pushes, is
+    bit-identical to the oracle's push of the same pre-step state (cell
+    index after move_p, offsets, momenta, weight), particles matched one for
+    one by their upload index; FP32 field energy within 1e-3 of FP64."""
     h = {}
-    for dtype in (np.float32, np.float64):
+    for dtype, nchk in ((np.float32, 200), (np.float64, 50)):
         deck = decks.two_stream_3d(dtype)
-        h[dtype] = _run_checked(deck, (1, 2, 33, 200), deck.num_steps)
+        sp = deck.step_params(energy_interval=1)
+        hist = np.zeros((deck.num_steps, 3))
+        with load(deck) as sim:
+            p = sim.download_particles()
+            t = sim.download_particle_tags()
+            for s in range(nchk):
+                f0 = sim.download_fields()
+                hist[s] = sim.step(sp, 1)[0]
+                got = sim.download_particles()
+                tags = sim.download_particle_tags()
+                assert sim.device_errors() == 0
+                ref = oracle_push(deck, f0, p)
+                _assert_same_by_tag(got, tags, ref, t, f"{_name(dtype)} step {s + 1}")
+                p, t = got, tags
+                if (s + 1) % 25 == 0:
+                    print(f"{deck.name} {_name(dtype)}: {s + 1} pushes checked", flush=True)
+            if deck.num_steps > nchk:
+                hist[nchk:] = sim.step(sp, deck.num_steps - nchk)
+            assert sim.device_errors() == 0
+        h[dtype] = hist
     e32 = h[np.float32][:, 0] + h[np.float32][:, 1]
     e64 = h[np.float64][:, 0] + h[np.float64][:, 1]
     rel = np.abs(e32 - e64) / e64
