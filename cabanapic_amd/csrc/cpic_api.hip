@@ -70,6 +70,11 @@ constexpr int kCompactInterval = 32;
 
 }  // namespace
 
+namespace cpic_detail {
+// the error path of the library's other translation units
+int fail_msg(int code, const char* msg) { return fail(code, "%s", msg); }
+}  // namespace cpic_detail
+
 // The particle columns and everything sized by them (see
 // cpic_push_kernels.h): replaced as a whole when the capacity changes.
 struct ColumnSet {

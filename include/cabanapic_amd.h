@@ -312,6 +312,23 @@ int cpic_tangent_last_push_movers(cpic_tangent *tg, int64_t *movers);
  * every stage as it goes.  Steps with stage timing on always launch. */
 int cpic_tangent_set_graph(cpic_tangent *tg, int32_t enabled);
 
+/* ---- allocation-free variant -------------------------------------------- */
+
+/* cpic_energies for caller-owned device fields, with no device allocation:
+ * no counterpart in the reference; for callers that cannot call hipMalloc
+ * (code under HIP graph capture, serving loops, harnesses that forbid it).
+ * fields: device pointer to 9 x Ng values of real_size bytes (the host
+ * layout above); out: device pointer to 2 doubles, receiving the E and B
+ * energies of src/fields.h:556-615 (0.5 * sum of squares over interior
+ * cells, FP64), bit-identical to cpic_energies on the same fields.
+ * Asynchronous on stream (a hipStream_t, NULL for the default stream).  The
+ * reduction's block partials live in a static 64 KiB array per GPU (at most
+ * 4096 blocks, whatever the grid); calls that overlap on one GPU share it
+ * and race.  Grids beyond 32-bit voxel indexing are rejected. */
+int cpic_field_energies_noalloc(const void *fields, int32_t real_size,
+                                int64_t nx, int64_t ny, int64_t nz,
+                                double *out, void *stream);
+
 /* ---- deck support (host only, no device call) -------------------------- */
 
 /* Synthetic plasma loader for the decks of the benchmark and test
