@@ -73,6 +73,12 @@ SIGNATURES = {
     "cpic_download_particle_tags": (ctypes.c_int, [_P, _I64, _P]),
     "cpic_column_rows": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "cpic_field_energies_noalloc": (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _P]),
+    "cpic_adjoint_create": (ctypes.c_int, [_P, _PP]),
+    "cpic_adjoint_destroy": (ctypes.c_int, [_P]),
+    "cpic_forward": (ctypes.c_int, [_P, ctypes.POINTER(StepParams), _I64, _I32, _P]),
+    "cpic_backward": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "cpic_adjoint_memory": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64),
+                                           ctypes.POINTER(_I64)]),
     "cpic_upload_fields": (ctypes.c_int, [_P, _P]),
     "cpic_download_fields": (ctypes.c_int, [_P, _P]),
     "cpic_upload_interpolators": (ctypes.c_int, [_P, _P]),

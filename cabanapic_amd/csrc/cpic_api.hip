@@ -17,6 +17,7 @@
 #include "../../include/cabanapic_amd.h"
 #include "cpic_push_kernels.h"
 #include "cpic_tangent_kernels.h"
+#include "cpic_adjoint_kernels.h"
 
 using namespace cpic;
 
@@ -619,6 +620,50 @@ int check_headroom(cpic_context* c) {
                 c->occ_host[1]);
   if (c->D == 0 || c->occ_host[0] < c->D - std::max(8, c->D / 4)) return CPIC_OK;
   return with_real(c, [&](auto r) { return regrow_columns<decltype(r)>(c); });
+}
+
+}  // namespace
+
+namespace {
+
+// nsteps iterations of the loop body of example/example.cpp:216-279 on the
+// context's stream, no host synchronisation; energies of step s (1-based)
+// into dhist row s - 1 (3 doubles: E, B, kinetic) when dhist is not null,
+// as sp's energy and kinetic intervals ask.
+template <class R>
+int run_steps(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, double* dhist) {
+  const int ei = sp->energy_interval;
+  const int ki = sp->kinetic_interval;
+  const R px = (R)sp->px, py = (R)sp->py, pz = (R)sp->pz;
+  const R half = (R)0.5;
+  const R hpx = half * px, hpy = half * py, hpz = half * pz;
+  for (int64_t step = 1; step <= nsteps; ++step) {
+    int t;
+    if ((t = stage_load_interpolator<R>(c))) return t;
+    // the push overwrites every interior accumulator, so the clear is only
+    // needed while ghost accumulators may hold data
+    if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
+    if ((t = check_headroom(c))) return t;
+    const int ci = sp->compact_interval > 0 ? sp->compact_interval : kCompactInterval;
+    const int compact = (c->pushes++ % ci == 0) ? 1 : 0;
+    if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
+                           (R)sp->qsp, 0, compact)))
+      return t;
+    if ((t = note_occupancy(c))) return t;
+    if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;
+    if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
+    if ((t = stage_advance_e<R>(c, px, py, pz, (R)sp->dt_eps0))) return t;
+    if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
+    const bool do_e = ei > 0 && step % ei == 0, do_k = ki > 0 && step % ki == 0;
+    if (do_e || do_k) {
+      StageTimer tm(c, CPIC_STAGE_ENERGY);
+      double* row = dhist ? dhist + 3 * (step - 1) : c->red_out;
+      if (do_e && (t = launch_field_energy<R>(c, row))) return t;
+      if (do_k && (t = launch_kinetic<R>(c, (double)sp->me * sp->c * sp->c, row + 2))) return t;
+      if ((t = tm.finish())) return t;
+    }
+  }
+  return CPIC_OK;
 }
 
 }  // namespace
@@ -1252,39 +1297,7 @@ int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, doubl
       }
       dhist = c->hist_dev;
     }
-    int s = with_real(c, [&](auto r) {
-      using R = decltype(r);
-      const R px = (R)sp->px, py = (R)sp->py, pz = (R)sp->pz;
-      const R half = (R)0.5;
-      const R hpx = half * px, hpy = half * py, hpz = half * pz;
-      for (int64_t step = 1; step <= nsteps; ++step) {
-        int t;
-        if ((t = stage_load_interpolator<R>(c))) return t;
-        // the push overwrites every interior accumulator, so the clear is only
-        // needed while ghost accumulators may hold data
-        if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
-        if ((t = check_headroom(c))) return t;
-        const int ci = sp->compact_interval > 0 ? sp->compact_interval : kCompactInterval;
-        const int compact = (c->pushes++ % ci == 0) ? 1 : 0;
-        if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
-                               (R)sp->qsp, 0, compact)))
-          return t;
-        if ((t = note_occupancy(c))) return t;
-        if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;
-        if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
-        if ((t = stage_advance_e<R>(c, px, py, pz, (R)sp->dt_eps0))) return t;
-        if ((t = stage_advance_b<R>(c, hpx, hpy, hpz))) return t;
-        const bool do_e = ei > 0 && step % ei == 0, do_k = ki > 0 && step % ki == 0;
-        if (do_e || do_k) {
-          StageTimer tm(c, CPIC_STAGE_ENERGY);
-          double* row = dhist ? dhist + 3 * (step - 1) : c->red_out;
-          if (do_e && (t = launch_field_energy<R>(c, row))) return t;
-          if (do_k && (t = launch_kinetic<R>(c, (double)sp->me * sp->c * sp->c, row + 2))) return t;
-          if ((t = tm.finish())) return t;
-        }
-      }
-      return CPIC_OK;
-    });
+    int s = with_real(c, [&](auto r) { return run_steps<decltype(r)>(c, sp, nsteps, dhist); });
     if (s == CPIC_OK && dhist) {
       std::vector<double> h(3 * nsteps);
       hipError_t e = hipMemcpyAsync(h.data(), dhist, sizeof(double) * 3 * nsteps,
@@ -1364,3 +1377,4 @@ int cpic_device_errors(cpic_context* c, int32_t* flags) {
 }  // extern "C"
 
 #include "cpic_tangent.inc"
+#include "cpic_adjoint.inc"

@@ -312,6 +312,46 @@ int cpic_tangent_last_push_movers(cpic_tangent *tg, int64_t *movers);
  * every stage as it goes.  Steps with stage timing on always launch. */
 int cpic_tangent_set_graph(cpic_tangent *tg, int32_t enabled);
 
+/* ---- reverse-mode derivative (adjoint) ---------------------------------- */
+
+/* The reference has no derivative of its step; these entry points add the
+ * reverse mode, as a forward()/backward() pair on a primal context.
+ * cpic_forward runs nsteps steps of cpic_step on the primal context and
+ * keeps its state (particles, fields) every checkpoint_interval steps;
+ * cpic_backward then takes the derivative of an objective with respect to
+ * the final state (seed_fields: 9 x Ng doubles in the field layout;
+ * seed_particles: 6 x n doubles, derivatives with respect to x, y, z, ux,
+ * uy, uz of each particle in upload order, or NULL for none) and returns
+ * the derivative with respect to the initial state (grad_fields, 9 x Ng;
+ * grad_particles, 6 x n in upload order; either may be NULL).  Each segment
+ * of checkpoint_interval steps is recomputed from its checkpoint, keeping
+ * the state before each of its steps, and transposed step by step: the
+ * transposes of src/fields.h:618-719 (Yee updates and ghost handling),
+ * src/accumulator.cpp:45-122 (unload), src/interpolator.cpp:4-124 (field
+ * gather) and src/push.h:124-256 with src/move_p.h:93-209 (push and
+ * deposit, face crossings held as taken), in FP64 whatever real_t is.  It is
+ * the transpose of the linearization cpic_tangent_step computes.  A
+ * checkpoint re-lays the particle columns out as a later recompute will,
+ * so the recompute repeats the forward run bit for bit.  The derivative
+ * with respect to a deck parameter is the dot product of the returned
+ * initial-state derivative with the initial state's derivative with respect
+ * to that parameter.  After cpic_backward the primal context holds the final
+ * state again.  Device memory: (ceil(nsteps / K) + 1 + K) snapshots of
+ * 32 n + 36 Ng bytes (float; 60 n + 72 Ng in double), see
+ * cpic_adjoint_memory.  EM field solver only.  Synchronous. */
+typedef struct cpic_adjoint cpic_adjoint;
+int cpic_adjoint_create(cpic_context *primal, cpic_adjoint **out);
+int cpic_adjoint_destroy(cpic_adjoint *adj);
+int cpic_forward(cpic_adjoint *adj, const cpic_step_params *sp, int64_t nsteps,
+                 int32_t checkpoint_interval, double *hist);
+int cpic_backward(cpic_adjoint *adj, const double *seed_fields,
+                  const double *seed_particles, double *grad_fields,
+                  double *grad_particles);
+/* Bytes of device memory held for the checkpoints (with the final state),
+ * for the snapshots of one segment, and for the adjoint arrays. */
+int cpic_adjoint_memory(cpic_adjoint *adj, int64_t *checkpoint_bytes,
+                        int64_t *segment_bytes, int64_t *adjoint_bytes);
+
 /* ---- allocation-free variant -------------------------------------------- */
 
 /* cpic_energies for caller-owned device fields, with no device allocation:

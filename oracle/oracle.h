@@ -79,7 +79,16 @@ extern "C" {
   long ora_run##S(R *fields, R *interp, R *acc, R *x, R *y, R *z, R *ux,       \
                   R *uy, R *uz, const R *w, int32_t *cell, long np, long nx,   \
                   long ny, long nz, const R *consts, long nsteps,              \
-                  int nthreads, double *hist);
+                  int nthreads, double *hist);                                 \
+  void ora_step_adjoint##S(const R *fields, const R *x, const R *y,            \
+                           const R *z, const R *ux, const R *uy, const R *uz,  \
+                           const R *w, const int32_t *cell, long np, long nx,  \
+                           long ny, long nz, const R *consts, double *lf,      \
+                           const double *lp_out, double *lp_in);               \
+  void ora_stage_adjoint##S(int which, double *lf, double *lacc,               \
+                            const double *li, long nx, long ny, long nz,       \
+                            double px, double py, double pz, double c0,        \
+                            double c1, double c2);
 
 ORA_DECLARE(float, _f)
 ORA_DECLARE(double, _d)

@@ -11,3 +11,4 @@
 #define S _d
 #include "oracle_impl.inc"
 #include "oracle_deck.inc"
+#include "oracle_adjoint.inc"

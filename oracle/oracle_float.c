@@ -11,3 +11,4 @@
 #define S _f
 #include "oracle_impl.inc"
 #include "oracle_deck.inc"
+#include "oracle_adjoint.inc"

@@ -20,3 +20,4 @@
 #define sqrtf sqrt
 #include "oracle_impl.inc"
 #include "oracle_deck.inc"
+#include "oracle_adjoint.inc"

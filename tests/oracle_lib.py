@@ -77,6 +77,8 @@ def _declare(L):
         g("ora_run").argtypes = [P] * 10 + [P, Lg, Lg, Lg, Lg, P, Lg, I, P]
         g("ora_run").restype = Lg
         g("ora_deck_2stream_em").argtypes = [P]
+        g("ora_step_adjoint").argtypes = [P] * 9 + [Lg, Lg, Lg, Lg, P, P, P, P]
+        g("ora_stage_adjoint").argtypes = [I, P, P, P, Lg, Lg, Lg] + [ctypes.c_double] * 6
 
 
 def _p(a):
@@ -189,6 +191,20 @@ class OracleState:
     def kinetic_energy(self, me=1.0, c=1.0):
         return self.fn("ora_kinetic_energy")(_p(self.p["ux"]), _p(self.p["uy"]), _p(self.p["uz"]),
                                              _p(self.p["w"]), self.np, me, c)
+
+    def step_adjoint(self, consts, lf, lp_out):
+        """Adjoint of one step (ora_step_adjoint) around this state, which is
+        left unchanged: lf (9 x Ng doubles), the adjoint of the fields after
+        the step, becomes that before it; lp_out (6 x np doubles) is the
+        adjoint of the particles after the step; returns that before it."""
+        k = np.asarray(consts, self.dtype)
+        assert k.size == NCONSTS and lf.dtype == np.float64 and lf.size == 9 * self.ng
+        lp_out = np.ascontiguousarray(lp_out, np.float64)
+        assert lp_out.size == 6 * self.np
+        lp_in = np.zeros((6, self.np))
+        self.fn("ora_step_adjoint")(_p(self.fields), *self._pp(), self.np, self.nx, self.ny,
+                                    self.nz, _p(k), _p(lf), _p(lp_out), _p(lp_in))
+        return lp_in
 
     def run(self, consts, nsteps, nthreads=1, hist=True):
         k = np.asarray(consts, self.dtype)
