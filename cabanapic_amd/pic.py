@@ -357,6 +357,78 @@ class TangentSimulation:
         check(self.lib.cpic_tangent_set_graph(self._h, 1 if enabled else 0))
 
 
+class Adjoint:
+    """Reverse-mode derivative of a run of steps on a Simulation (the
+    cpic_forward / cpic_backward pair): forward() runs the steps keeping a
+    checkpoint every `interval` steps; backward() takes the derivative of an
+    objective with respect to the final state and returns it with respect to
+    the initial state."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        self.lib = sim.lib
+        h = ctypes.c_void_p()
+        check(self.lib.cpic_adjoint_create(sim._h, ctypes.byref(h)))
+        self._h = h
+        self._n = 0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.cpic_adjoint_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def forward(self, params, nsteps, interval, history=True):
+        """Runs nsteps steps; returns the (nsteps, 3) energy history [E, B,
+        kinetic] as params' intervals ask (NaN elsewhere), or None."""
+        if isinstance(params, dict):
+            params = make_step_params(**params)
+        hist = np.full((int(nsteps), 3), np.nan) if history else None
+        check(self.lib.cpic_forward(self._h, ctypes.byref(params), int(nsteps), int(interval),
+                                    _ptr(hist) if history else None))
+        self._n = self.sim.num_particles()
+        return hist
+
+    def backward(self, seed_fields, seed_particles=None):
+        """seed_fields: (9, Ng) derivative of the objective with respect to
+        the final fields; seed_particles: (6, n) with respect to the final
+        x, y, z, ux, uy, uz of each particle in upload order (or None).
+        Returns (grad_fields (9, Ng), grad_particles (6, n)) with respect to
+        the initial state."""
+        sf = np.ascontiguousarray(seed_fields, np.float64)
+        if sf.size != 9 * self.sim.ncells:
+            raise ValueError("seed_fields must hold 9 x Ng values")
+        sp = None
+        if seed_particles is not None:
+            sp = np.ascontiguousarray(seed_particles, np.float64)
+            if sp.size != 6 * self._n:
+                raise ValueError("seed_particles must hold 6 x n values")
+        gf = np.empty((9, self.sim.ncells))
+        gp = np.empty((6, self._n))
+        check(self.lib.cpic_backward(self._h, _ptr(sf), _ptr(sp) if sp is not None else None,
+                                     _ptr(gf), _ptr(gp)))
+        return gf, gp
+
+    def memory(self):
+        """Device bytes: checkpoints (with the final state), one segment's
+        snapshots, the adjoint arrays."""
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(self.lib.cpic_adjoint_memory(self._h, ctypes.byref(a), ctypes.byref(b),
+                                           ctypes.byref(c)))
+        return {"checkpoints": a.value, "segment": b.value, "adjoint": c.value}
+
+
 def make_step_params(qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, dx, dy, dz, dt, px, py, pz,
                      dt_eps0, me=1.0, c=1.0, boundary=BOUNDARY_PERIODIC, energy_interval=1,
                      compact_interval=0, kinetic_interval=0):
