@@ -129,6 +129,90 @@ def differentiation_config_costs(steps):
     return 1e3 * t_fwd / steps, 1e3 * t_tan / steps
 
 
+def adjoint_costs(steps, interval):
+    """Reverse-mode derivative on the differentiation configuration (3D
+    two-stream, 32^3 cells, 2 x 16 ppc, FP32 primal, FP64 adjoint) of the
+    final E energy: ms per step of the plain forward run (cpic_step), of
+    cpic_forward with a checkpoint every `interval` steps, and of
+    cpic_backward (recompute and transpose), all from the same initial
+    state, and the gradient with respect to the beam drift."""
+    from cabanapic_amd import Adjoint, Simulation, decks
+    d = decks.two_stream_3d(np.float32, n=32, nppc=32, steps=steps)
+    parts = d.load_particles()
+    sp = d.step_params(energy_interval=1)
+    members = ("x", "y", "z", "ux", "uy", "uz", "w")
+    sim = Simulation(d.nx, d.ny, d.nz, d.num_particles, np.float32)
+
+    def upload():
+        sim.upload_particles(*[parts[m] for m in members], parts["cell"])
+        sim.upload_fields(np.zeros((9, d.num_cells), np.float32))
+
+    upload()
+    sim.step(sp, 10)  # warm-up
+    upload()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(sp, steps)
+    t_plain = time.perf_counter() - t0
+    upload()
+    with Adjoint(sim) as adj:
+        t0 = time.perf_counter()
+        adj.forward(sp, steps, interval)
+        t_fwd = time.perf_counter() - t0
+        f = sim.download_fields().astype(np.float64)
+        seed = np.zeros_like(f)
+        inner = np.zeros((d.nz + 2, d.ny + 2, d.nx + 2), bool)
+        inner[1:-1, 1:-1, 1:-1] = True
+        seed[:3, inner.ravel()] = f[:3, inner.ravel()]
+        t0 = time.perf_counter()
+        _, gp = adj.backward(seed)
+        t_bwd = time.perf_counter() - t0
+        mem = adj.memory()
+    errs = sim.device_errors()
+    sim.close()
+    if errs:
+        raise SystemExit(f"adjoint run: device error flags {errs}")
+    # d(ux)/d(v0) of the loader: ux = s d(v0) (1 + a sin) + uth n, linear in the drift
+    rel = 1e-6
+    lo = decks.two_stream_3d(np.float64, n=32, nppc=32, steps=steps, v0=0.2 * (1 - rel))
+    hi = decks.two_stream_3d(np.float64, n=32, nppc=32, steps=steps, v0=0.2 * (1 + rel))
+    dux = (hi.load_particles()["ux"] - lo.load_particles()["ux"]) / (2 * rel * 0.2)
+    grad = float(np.dot(gp[3], dux))
+    return (1e3 * t_plain / steps, 1e3 * t_fwd / steps, 1e3 * t_bwd / steps, grad, mem)
+
+
+def column_gradient(steps, interval):
+    """dA_3/dB_z on the 192 x 192 x 64 x 32 electron column: the m = 3
+    amplitude of the radial E field after `steps` FP32 steps, differentiated
+    with respect to the guide field by cpic_forward / cpic_backward with a
+    checkpoint every `interval` steps."""
+    from cabanapic_amd import Adjoint, Simulation, decks
+    d = decks.dioctron_3d(np.float32, steps=steps)
+    parts = d.load_particles()
+    sim = Simulation(d.nx, d.ny, d.nz, d.num_particles, np.float32)
+    sim.upload_particles(*[parts[m] for m in ("x", "y", "z", "ux", "uy", "uz", "w")],
+                         parts["cell"])
+    del parts
+    sim.upload_fields(d.load_fields())
+    sp = d.step_params(energy_interval=1)
+    with Adjoint(sim) as adj:
+        t0 = time.perf_counter()
+        adj.forward(sp, steps, interval)
+        t_fwd = time.perf_counter() - t0
+        amp, seed = decks.column_mode_amplitude(d, sim.download_fields())
+        t0 = time.perf_counter()
+        gf, gp = adj.backward(seed)
+        t_bwd = time.perf_counter() - t0
+        mem = adj.memory()
+    errs = sim.device_errors()
+    sim.close()
+    if errs:
+        raise SystemExit(f"column adjoint: device error flags {errs}")
+    tf, tp = decks.column_guide_field_tangent(d)
+    grad = float(np.vdot(gf, tf) + np.vdot(gp, tp))
+    return amp, grad, 1e3 * t_fwd / steps, 1e3 * t_bwd / steps, mem
+
+
 def cpu_description():
     """CPU model and physical core count of this host (from /proc/cpuinfo),
     and the cores this process may run on."""
@@ -206,6 +290,11 @@ def main():
                          "may use, within OMP_NUM_THREADS)")
     ap.add_argument("--tangent-steps", type=int, default=5,
                     help="timed steps of the tangent-linear step (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=500,
+                    help="steps of the reverse-mode run on the differentiation configuration "
+                         "(0 = skip)")
+    ap.add_argument("--column-gradient-steps", type=int, default=1000,
+                    help="steps of the electron column's dA_3/dB_z adjoint run (0 = skip)")
     ap.add_argument("--column-steps", type=int, default=20,
                     help="timed FP32 steps of the 192x192x64 x 32 ppc electron-column "
                          "configuration, reported beside the metric (0 = skip)")
@@ -294,6 +383,37 @@ def main():
         extra.update(diffconfig_forward_ms_per_step=round(f4, 4),
                      diffconfig_tangent_ms_per_step=round(t4, 4),
                      diffconfig_tangent_over_forward=round(t4 / f4, 3))
+
+    if args.adjoint_steps > 0:
+        k = 50
+        f_ms, af_ms, ab_ms, g, mem = adjoint_costs(args.adjoint_steps, k)
+        print(f"# adjoint, differentiation config 32^3 x 32, {args.adjoint_steps} steps, "
+              f"checkpoints every {k}: forward {f_ms:.3f} ms/step, forward with checkpoints "
+              f"{af_ms:.3f}, backward {ab_ms:.3f} ({(af_ms + ab_ms) / f_ms:.2f} x the forward "
+              f"run); dW_E/dv0 = {g:.6e}", file=sys.stderr, flush=True)
+        extra.update(adjoint_diffconfig_steps=args.adjoint_steps,
+                     adjoint_diffconfig_checkpoint_interval=k,
+                     adjoint_diffconfig_forward_ms_per_step=round(f_ms, 4),
+                     adjoint_diffconfig_checkpointed_forward_ms_per_step=round(af_ms, 4),
+                     adjoint_diffconfig_backward_ms_per_step=round(ab_ms, 4),
+                     adjoint_diffconfig_backward_over_forward=round(ab_ms / f_ms, 3),
+                     adjoint_diffconfig_total_over_forward=round((af_ms + ab_ms) / f_ms, 3),
+                     adjoint_diffconfig_dWE_dv0=g,
+                     adjoint_diffconfig_hbm_bytes=mem)
+
+    if args.column_gradient_steps > 0:
+        k = max(1, int(round(args.column_gradient_steps ** 0.5)))
+        amp, g, cf_ms, cb_ms, mem = column_gradient(args.column_gradient_steps, k)
+        print(f"# electron column dA_3/dB_z after {args.column_gradient_steps} FP32 steps "
+              f"(checkpoints every {k}): A_3 = {amp:.6e}, dA_3/dB_z = {g:.6e}; forward "
+              f"{cf_ms:.3f} ms/step, backward {cb_ms:.3f} ms/step; HBM {mem}", file=sys.stderr,
+              flush=True)
+        extra.update(column_gradient_steps=args.column_gradient_steps,
+                     column_gradient_checkpoint_interval=k,
+                     column_mode3_amplitude=amp, column_dA3_dBz=g,
+                     column_gradient_forward_ms_per_step=round(cf_ms, 4),
+                     column_gradient_backward_ms_per_step=round(cb_ms, 4),
+                     column_gradient_hbm_bytes=mem)
 
     if args.column_steps > 0:
         dc = decks.dioctron_3d(np.float32)

@@ -259,6 +259,63 @@ def dioctron_3d(dtype=np.float32, nx=192, ny=192, nz=64, nppc=32, steps=1000, se
     return d.derive_params()
 
 
+def column_mode_amplitude(deck, fields, mode=3):
+    """Amplitude of the azimuthal mode `mode` of the radial electric field of
+    an electron-column deck, about the column axis (the box centre in x, y):
+        A = |sum over interior voxels of E_r(v) exp(-i mode theta(v))| / nz,
+    with ex and ey each taken at its own Yee point (ex on the x edge at the
+    (y-, z-) corner of voxel v, ey on the y edge at its (x-, z-) corner) and
+    E_r = ex cos(theta) + ey sin(theta) there.  Returns A and dA/dfields
+    (9 x Ng, FP64), the seed of its adjoint."""
+    f = np.asarray(fields, np.float64).reshape(9, -1)
+    nx, ny, nz = deck.nx, deck.ny, deck.nz
+    dx, dy = float(deck.dx), float(deck.dy)
+    X, Y = nx + 2, ny + 2
+    ix = np.arange(1, nx + 1)
+    iy = np.arange(1, ny + 1)
+    seed = np.zeros_like(f)
+    C = S = 0.0
+    parts = []
+    for comp, (ox, oy) in ((0, (0.5, 0.0)), (1, (0.0, 0.5))):
+        xx = ((ix[None, :] - 1 + ox) * dx) - 0.5 * nx * dx
+        yy = ((iy[:, None] - 1 + oy) * dy) - 0.5 * ny * dy
+        th = np.arctan2(yy, xx)
+        proj = np.cos(th) if comp == 0 else np.sin(th)
+        gc = proj * np.cos(mode * th)  # (ny, nx)
+        gs = -proj * np.sin(mode * th)
+        vox = (ix[None, :] + X * iy[:, None])[None, :, :] + X * Y * np.arange(1, nz + 1)[:, None, None]
+        e = f[comp][vox]  # (nz, ny, nx)
+        C += float((e * gc[None]).sum())
+        S += float((e * gs[None]).sum())
+        parts.append((comp, vox, gc, gs))
+    A = np.hypot(C, S) / nz
+    for comp, vox, gc, gs in parts:
+        seed[comp][vox] = np.broadcast_to((C * gc + S * gs)[None] / (A * nz * nz), vox.shape)
+    return A, seed
+
+
+def column_guide_field_tangent(deck, rel=1e-6):
+    """The initial state's derivative with respect to the guide field B_z of
+    an electron-column deck: cbz = B_z in every voxel (fields), and the
+    E x B drift momenta of the particles, by central differences of the
+    FP64 loader (positions and weights do not depend on B_z).  Returns
+    (fields tangent (9, Ng), particle tangent (6, n))."""
+    bz = deck.plasma["guide_bz"]
+    vals = []
+    for sgn in (1, -1):
+        d = dioctron_3d(np.float64, nx=deck.nx, ny=deck.ny, nz=deck.nz, nppc=deck.nppc,
+                        steps=deck.num_steps, seed=deck.plasma["seed"], bz=bz * (1 + sgn * rel),
+                        amp=deck.plasma["pert_amp"], mode=deck.plasma["pert_mode"],
+                        dx=float(deck.dx))
+        vals.append(d.load_particles())
+    t = np.zeros((6, vals[0]["cell"].size))
+    for k, m in ((3, "ux"), (4, "uy"), (5, "uz")):
+        t[k] = (vals[0][m] - vals[1][m]) / (2 * rel * bz)
+    tf = np.zeros((9, deck.num_cells))
+    tf[5] = 1.0
+    return tf, t
+
+
 CONFIGS = {
     "twostream1d": two_stream_1d,
     "weibel128": weibel_3d,

@@ -193,3 +193,53 @@ def test_differentiation_config_adjoint_equals_tangent():
     tang, adjg = out[F64]
     for p, _ in params:
         assert abs(adjg[p] - tang[p]) <= 1e-8 * abs(tang[p]), p
+
+
+def _column_run(deck, nsteps):
+    """Final fields of nsteps FP64 steps of an electron-column deck."""
+    parts = deck.load_particles()
+    with Simulation(deck.nx, deck.ny, deck.nz, deck.num_particles, deck.R) as sim:
+        sim.upload_particles(*[parts[m] for m in MEMBERS], parts["cell"])
+        sim.upload_fields(deck.load_fields())
+        sim.step(deck.step_params(energy_interval=0), nsteps, history=False)
+        return sim.download_fields()
+
+
+@pytest.mark.timeout(600)
+def test_column_mode_gradient_equals_tangent():
+    """A reduced electron column (48 x 48 x 8 cells, 8 ppc, guide field
+    B_z = 1, m = 3 perturbation), 60 FP64 steps with checkpoints every 16:
+    the adjoint's dA_3/dB_z of the m = 3 amplitude of the radial E field
+    equals the tangent-linear step's to 1e-8; central differences of the
+    forward run (B_z +- 1e-6 relative) are printed beside them."""
+    nsteps = 60
+    deck = decks.dioctron_3d(F64, nx=48, ny=48, nz=8, nppc=8, steps=nsteps)
+    parts = deck.load_particles()
+    fields = deck.load_fields()
+    tf, tp = decks.column_guide_field_tangent(deck)
+    sp = deck.step_params(energy_interval=1)
+    with Simulation(deck.nx, deck.ny, deck.nz, deck.num_particles, F64) as sim, \
+            Adjoint(sim) as adj:
+        sim.upload_particles(*[parts[m] for m in MEMBERS], parts["cell"])
+        sim.upload_fields(fields)
+        adj.forward(sp, nsteps, 16)
+        A, seed = decks.column_mode_amplitude(deck, sim.download_fields())
+        gf, gp = adj.backward(seed)
+        assert sim.device_errors() == 0
+    g_adj = float(np.vdot(gf, tf) + np.vdot(gp, tp))
+    with TangentSimulation(deck.nx, deck.ny, deck.nz, deck.num_particles, F64) as ts:
+        ts.primal.upload_fields(fields)
+        ts.tangent.upload_fields(tf)
+        ts.upload_particles(*[parts[m] for m in MEMBERS], parts["cell"], tp)
+        ts.step(sp, nsteps)
+        g_tan = float(np.vdot(seed, ts.tangent.download_fields()))
+    eps = 1e-6
+    amps = []
+    for sgn in (1, -1):
+        d = decks.dioctron_3d(F64, nx=48, ny=48, nz=8, nppc=8, steps=nsteps, bz=1.0 + sgn * eps)
+        amps.append(decks.column_mode_amplitude(d, _column_run(d, nsteps))[0])
+    fd = (amps[0] - amps[1]) / (2 * eps)
+    rel = abs(g_adj - g_tan) / abs(g_tan)
+    print(f"A_3 = {A:.6e}; dA_3/dB_z: adjoint {g_adj:.12e}, tangent {g_tan:.12e} (rel {rel:.2e}), "
+          f"central differences {fd:.6e}")
+    assert rel < 1e-8
