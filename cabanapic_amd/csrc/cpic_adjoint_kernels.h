@@ -11,15 +11,15 @@
 // root), crossing decisions held as taken and a position clamped to a face
 // given zero derivative.  Adjoints are FP64 whatever real_t is.
 //
-// The particle adjoint is computed owner-computes, like the forward push:
-// one lane per cell walks the particles that started the step in its cell
-// (from a snapshot of the step's input, see k_snapshot_columns), recomputes
-// each particle's push and streaks, and takes the transpose in reverse
-// order.  The deposit's transpose is a gather (each streak reads the
-// accumulator adjoint of its cell), and the transpose of the field gather
-// (the interpolator adjoint) sums over the cell's own particles in the
-// lane's registers, so no atomics are needed.  Particle adjoints are indexed
-// by the particle's tag (its upload index).
+// The particle adjoint runs one thread per particle over a snapshot of the
+// step's input (k_snapshot_columns: whole slots, cells in order, so a
+// cell's particles sit in consecutive lanes).  Each thread recomputes its
+// particle's push and streaks and takes the transpose in reverse order.  The
+// deposit's transpose is a gather (each streak reads the accumulator adjoint
+// of its cell); the transpose of the field gather (the interpolator adjoint)
+// is summed over each run of lanes in one cell and added once per run.
+// Particle adjoints are records indexed by the particle's tag (its upload
+// index).
 //
 // The field stages are linear; their transposes are written as gathers: each
 // output voxel sums what the forward stage's outputs read from it.
@@ -31,9 +31,9 @@ namespace cpic {
 
 // ---------------------------------------------------------------------------
 // Snapshots of the particle columns: cells in interior order, a cell's
-// particles in row order, as SoA x, y, z, ux, uy, uz, w (stride n) and the
-// tag (int32); first[ci] = position of cell ci's first particle,
-// first[nc] = n.
+// particles in row order, each as its whole column slot (x, y, z, ux, uy, uz,
+// w, tag), with the interior cell of every slot; first[ci] = position of
+// cell ci's first particle, first[nc] = n.
 
 // Exclusive prefix sum of cnt[0..n) into out[0..n], out[n] = total, in three
 // passes (block scans, a scan of the block totals, the block offsets added).
@@ -104,12 +104,14 @@ k_scan_add(int* __restrict__ out, int n, const int* __restrict__ sums) {
     if (base + k < n) out[base + k] += add;
 }
 
+// The context's columns into a snapshot: lane per cell, each occupied row
+// (in row order) copied as its whole slot to slots[first[ci] + k]; cell[]
+// receives the interior cell index of every snapshot slot.
 template <class R>
 __global__ void __launch_bounds__(kBlock)
 k_snapshot_columns(const R* __restrict__ cols, const int* __restrict__ cnt,
                    const unsigned long long* __restrict__ mask, int nc, int nc_pad, int D,
-                   const int* __restrict__ first, long n, R* __restrict__ soa,
-                   int* __restrict__ tag) {
+                   const int* __restrict__ first, R* __restrict__ slots, int* __restrict__ cell) {
   typedef typename Vec4T<R>::type V4;
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
   if (ci >= nc) return;
@@ -121,53 +123,63 @@ k_snapshot_columns(const R* __restrict__ cols, const int* __restrict__ cnt,
     if ((j & 63) == 0) m = mask[(size_t)(j >> 6) * nc_pad + ci];
     if (!((m >> (j & 63)) & 1ull)) continue;
     const V4* s = reinterpret_cast<const V4*>(cols + col_slot(ci, j, D) * kSlotLen);
-    const V4 a = s[0], b = s[1];
-    soa[o] = a.x;
-    soa[n + o] = a.y;
-    soa[2 * n + o] = a.z;
-    soa[3 * n + o] = a.w;
-    soa[4 * n + o] = b.x;
-    soa[5 * n + o] = b.y;
-    soa[6 * n + o] = b.z;
-    tag[o] = (int)TagBits<R>::get(b.w);
+    V4* d = reinterpret_cast<V4*>(slots + (size_t)o * kSlotLen);
+    d[0] = s[0];
+    d[1] = s[1];
+    cell[o] = ci;
     ++o;
   }
 }
 
-// A snapshot back into packed columns (rows 0 .. count-1 of every cell).
+// A snapshot back into packed columns: thread t < n moves snapshot slot t
+// to row (t - first[cell]) of its cell; thread t < nc_pad sets the count of
+// cell t.
 template <class R>
 __global__ void __launch_bounds__(kBlock)
-k_restore_columns(const R* __restrict__ soa, const int* __restrict__ tag,
+k_restore_columns(const R* __restrict__ slots, const int* __restrict__ cell,
                   const int* __restrict__ first, long n, int nc, int nc_pad, int D,
                   R* __restrict__ cols, int* __restrict__ cnt, int* __restrict__ err) {
   typedef typename Vec4T<R>::type V4;
-  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ci >= nc_pad) return;
-  if (ci >= nc) {
-    cnt[ci] = 0;
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nc_pad) cnt[t] = t < nc ? min(first[t + 1] - first[t], D) : 0;
+  if (t >= n) return;
+  const int ci = cell[t];
+  const long rank = t - first[ci];
+  if (rank >= D) {
+    atomicOr(err, kErrColumnFull);
     return;
   }
-  const long a = first[ci], b = first[ci + 1];
-  int k = 0;
-  for (long o = a; o < b; ++o, ++k) {
-    if (k >= D) {
-      atomicOr(err, kErrColumnFull);
-      break;
-    }
-    V4 x, y;
-    x.x = soa[o];
-    x.y = soa[n + o];
-    x.z = soa[2 * n + o];
-    x.w = soa[3 * n + o];
-    y.x = soa[4 * n + o];
-    y.y = soa[5 * n + o];
-    y.z = soa[6 * n + o];
-    y.w = TagBits<R>::make(tag[o]);
-    V4* d = reinterpret_cast<V4*>(cols + col_slot(ci, k, D) * kSlotLen);
-    d[0] = x;
-    d[1] = y;
+  const V4* s = reinterpret_cast<const V4*>(slots + (size_t)t * kSlotLen);
+  V4* d = reinterpret_cast<V4*>(cols + col_slot(ci, (int)rank, D) * kSlotLen);
+  d[0] = s[0];
+  d[1] = s[1];
+}
+
+// Particle adjoints are records of kAdjRec doubles per tag: x, y, z, ux,
+// uy, uz and two zeros, so that a record is two whole 32-byte sectors.
+constexpr int kAdjRec = 8;
+
+// [6][n] (host layout, upload order) <-> [n][kAdjRec] records
+__global__ void __launch_bounds__(kBlock)
+k_adj_soa_to_rec(const double* __restrict__ soa, long n, double* __restrict__ rec) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long)gridDim.x * blockDim.x) {
+    double* r = rec + (size_t)p * kAdjRec;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r[k] = soa[(size_t)k * n + p];
+    r[6] = 0;
+    r[7] = 0;
   }
-  cnt[ci] = k;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_adj_rec_to_soa(const double* __restrict__ rec, long n, double* __restrict__ soa) {
+  for (long p = (long)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+       p += (long)gridDim.x * blockDim.x) {
+    const double* r = rec + (size_t)p * kAdjRec;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) soa[(size_t)k * n + p] = r[k];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -434,15 +446,14 @@ __device__ __forceinline__ void boris_bwd(const V4& e0, const V4& e1, const V4& 
 }
 
 template <class R> struct AdjointPush {
-  const R* soa;        // [7][n] the step's input particles (snapshot)
-  const int* tag;      // [n]
-  const int* first;    // [nc + 1]
+  const R* slots;      // [n][kSlotLen] the step's input particles (snapshot)
+  const int* cell;     // [n] interior cell of each snapshot slot
   long n;
   const R* interp;     // the step's interpolators (cell-major, kInterpStride)
   const double* lacc;  // adjoint of the accumulators, slot-major [12][Ng]
-  const double* lout;  // [6][n]: adjoint of the step's output particles, by tag
-  double* lin;         // [6][n]: adjoint of its input particles, by tag
-  double* linterp;     // [Ng][kInterpStride]: adjoint of the interpolators
+  const double* lout;  // [n][kAdjRec]: adjoint of the step's output particles, by tag
+  double* lin;         // [n][kAdjRec]: adjoint of its input particles, by tag
+  double* linterp;     // [Ng][kInterpStride]: adjoint of the interpolators (added to)
   CellGrid g;
   int ncells;  // Ng
   R qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp;
@@ -456,113 +467,140 @@ __device__ __forceinline__ void load_lacc(const AdjointPush<R>& a, int v, double
   for (int k = 0; k < 12; ++k) l[k] = a.lacc[k * N + v];
 }
 
-// One lane per interior cell: every particle that started the step in the
-// cell, pushed again and transposed (src/push.h:124-256, src/move_p.h:93-209
-// in reverse).  Writes the input-particle adjoints by tag and the cell's
-// interpolator adjoint.
+// Sums l[18] over each run of consecutive lanes of the wave with the same
+// key (a segmented inclusive scan; the last lane of a run then adds the
+// run's total to linterp of voxel key).  Every lane of the wave calls it;
+// key < 0 for none.
+__device__ __forceinline__ void add_interp_runs(double* linterp, int key, double l[18]) {
+  const int lane = threadIdx.x & 63;
+  const int before = __shfl_up(key, 1, 64);
+  int head = (lane == 0 || before != key) ? 1 : 0;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int h = __shfl_up(head, off, 64);
+    const bool add = lane >= off && !head;
+#pragma unroll
+    for (int k = 0; k < 18; ++k) {
+      const double y = __shfl_up(l[k], off, 64);
+      if (add) l[k] += y;
+    }
+    if (lane >= off) head |= h;
+  }
+  const int after = __shfl_down(key, 1, 64);
+  if (key >= 0 && (lane == 63 || after != key)) {
+    double* li = linterp + (size_t)key * kInterpStride;
+#pragma unroll
+    for (int k = 0; k < 18; ++k)
+      if (l[k] != 0.0) unsafeAtomicAdd(li + k, l[k]);
+  }
+}
+
+// One thread per particle of the snapshot (cell-sorted, so the particles of
+// a cell sit in consecutive lanes): the particle's push and streaks
+// recomputed and transposed (src/push.h:124-256, src/move_p.h:93-209 in
+// reverse).  Writes the input-particle adjoint by tag; the interpolator
+// adjoints of the cell's particles are summed over the wave's runs of lanes
+// in one cell and added atomically.
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_push_adjoint(AdjointPush<R> a) {
-  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ci >= a.g.nc) return;
-  typedef typename Vec4T<R>::type V4;
-  const int v = interior_to_voxel(a.g, ci);
-  const V4* f = reinterpret_cast<const V4*>(a.interp + (size_t)v * kInterpStride);
-  const V4 e0 = f[0], e1 = f[1], e2 = f[2], b0 = f[3], b1 = f[4];
+  const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
   double lf[18];
 #pragma unroll
   for (int k = 0; k < 18; ++k) lf[k] = 0;
-  const R one = 1.;
-  const R one_third = 1. / 3.;
-  const long n = a.n;
-  const long p0 = a.first[ci], p1 = a.first[ci + 1];
-  for (long o = p0; o < p1; ++o) {
-    const R dx = a.soa[o], dy = a.soa[n + o], dz = a.soa[2 * n + o];
-    const R ux0 = a.soa[3 * n + o], uy0 = a.soa[4 * n + o], uz0 = a.soa[5 * n + o];
-    const R w = a.soa[6 * n + o];
-    const long t = a.tag[o];
-    if (t < 0 || t >= n) {
+  int key = -1;
+  if (p < a.n) {
+    typedef typename Vec4T<R>::type V4;
+    const int ci = a.cell[p];
+    const int v = interior_to_voxel(a.g, ci);
+    const V4* f = reinterpret_cast<const V4*>(a.interp + (size_t)v * kInterpStride);
+    const V4 e0 = f[0], e1 = f[1], e2 = f[2], b0 = f[3], b1 = f[4];
+    const V4* sl = reinterpret_cast<const V4*>(a.slots + (size_t)p * kSlotLen);
+    const V4 pa = sl[0], pb = sl[1];
+    const R one = 1.;
+    const R one_third = 1. / 3.;
+    const R dx = pa.x, dy = pa.y, dz = pa.z;
+    const R ux0 = pa.w, uy0 = pb.x, uz0 = pb.y, w = pb.z;
+    const long t = TagBits<R>::get(pb.w);
+    if (t < 0 || t >= a.n) {
       atomicOr(a.err, kErrBadCell);
-      continue;
-    }
-    // the forward push of this particle
-    R ux = ux0, uy = uy0, uz = uz0;
-    boris(e0, e1, e2, b0, b1, a.qdt_2mc, dx, dy, dz, ux, uy, uz);
-    R dd[3];
-    displacement(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, dd[0], dd[1], dd[2]);
-    const R mx = dx + dd[0], my = dy + dd[1], mz = dz + dd[2];
-    const R nx = mx + dd[0], ny = my + dd[1], nz = mz + dd[2];
-    const R q = w * a.qsp;
-    // adjoints of the outputs
-    double lpos[3], lu[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      lpos[k] = a.lout[(size_t)k * n + t];
-      lu[k] = a.lout[(size_t)(3 + k) * n + t];
-    }
-    double ld[3] = {0, 0, 0}, ldd[3] = {0, 0, 0};
-    double lacc[12];
-    if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
-      // in-cell: pos' = (d + dd) + dd, deposit with v5 = q ddx ddy ddz / 3
-      load_lacc(a, v, lacc);
-      const R t1 = q * dd[0], t2 = t1 * dd[1], t3 = t2 * dd[2];
-      double lm[3] = {0, 0, 0}, lv5 = 0;
-      streak_values_bwd(q, dd[0], dd[1], dd[2], mx, my, mz, lacc, ldd, lm, lv5);
-      const double lt3 = lv5 * (double)one_third;
-      const double lt2 = lt3 * (double)dd[2];
-      ldd[2] += lt3 * (double)t2;
-      const double lt1 = lt2 * (double)dd[1];
-      ldd[1] += lt2 * (double)t1;
-      ldd[0] += lt1 * (double)q;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        // m = d + dd, n = m + dd
-        const double lmk = lm[k] + lpos[k];
-        ldd[k] += lpos[k] + lmk;
-        ld[k] += lmk;
-      }
-      (void)t3;
     } else {
-      // the streaks: recount them forward, then transpose the last first
-      int K = 0;
-      {
-        R pos[3] = {dx, dy, dz}, rem[3] = {dd[0], dd[1], dd[2]};
-        R scratch[12];
-        for (; K < kMaxStreaks; ++K) {
-          if (streak(pos, rem, q, scratch) < 0) break;
-        }
-        ++K;
-        if (K > kMaxStreaks) {
-          atomicOr(a.err, kErrStreakLimit);
-          K = kMaxStreaks;
-        }
-      }
-      double lrem[3] = {0, 0, 0};  // the remainder after the last streak is no output
-      for (int s = K - 1; s >= 0; --s) {
-        // inputs and cell of streak s
-        R pos[3] = {dx, dy, dz}, rem[3] = {dd[0], dd[1], dd[2]};
-        int cs = ci;
-        R scratch[12];
-        for (int k = 0; k < s; ++k) cs = face_neighbor(a.g, cs, streak(pos, rem, q, scratch));
-        load_lacc(a, interior_to_voxel(a.g, cs), lacc);
-        streak_bwd(pos, rem, q, lacc, lpos, lrem);
-      }
+      key = v;
+      // the forward push of this particle
+      R ux = ux0, uy = uy0, uz = uz0;
+      boris(e0, e1, e2, b0, b1, a.qdt_2mc, dx, dy, dz, ux, uy, uz);
+      R dd[3];
+      displacement(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, dd[0], dd[1], dd[2]);
+      const R mx = dx + dd[0], my = dy + dd[1], mz = dz + dd[2];
+      const R nx = mx + dd[0], ny = my + dd[1], nz = mz + dd[2];
+      const R q = w * a.qsp;
+      // adjoints of the outputs
+      const double* lo = a.lout + (size_t)t * kAdjRec;
+      double lpos[3] = {lo[0], lo[1], lo[2]};
+      double lu[3] = {lo[3], lo[4], lo[5]};
+      double ld[3] = {0, 0, 0}, ldd[3] = {0, 0, 0};
+      double lacc[12];
+      if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
+        // in-cell: pos' = (d + dd) + dd, deposit with v5 = q ddx ddy ddz / 3
+        load_lacc(a, v, lacc);
+        const R t1 = q * dd[0], t2 = t1 * dd[1];
+        double lm[3] = {0, 0, 0}, lv5 = 0;
+        streak_values_bwd(q, dd[0], dd[1], dd[2], mx, my, mz, lacc, ldd, lm, lv5);
+        const double lt3 = lv5 * (double)one_third;
+        const double lt2 = lt3 * (double)dd[2];
+        ldd[2] += lt3 * (double)t2;
+        const double lt1 = lt2 * (double)dd[1];
+        ldd[1] += lt2 * (double)t1;
+        ldd[0] += lt1 * (double)q;
 #pragma unroll
-      for (int k = 0; k < 3; ++k) {
-        ld[k] = lpos[k];
-        ldd[k] = lrem[k];
-      }
-    }
-    displacement_bwd(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, ldd, lu);
-    boris_bwd(e0, e1, e2, b0, b1, a.qdt_2mc, dx, dy, dz, ux0, uy0, uz0, lu, ld, lf);
+        for (int k = 0; k < 3; ++k) {
+          // m = d + dd, n = m + dd
+          const double lmk = lm[k] + lpos[k];
+          ldd[k] += lpos[k] + lmk;
+          ld[k] += lmk;
+        }
+      } else {
+        // the streaks: recount them forward, then transpose the last first
+        int K = 0;
+        {
+          R pos[3] = {dx, dy, dz}, rem[3] = {dd[0], dd[1], dd[2]};
+          R scratch[12];
+          for (; K < kMaxStreaks; ++K) {
+            if (streak(pos, rem, q, scratch) < 0) break;
+          }
+          ++K;
+          if (K > kMaxStreaks) {
+            atomicOr(a.err, kErrStreakLimit);
+            K = kMaxStreaks;
+          }
+        }
+        double lrem[3] = {0, 0, 0};  // the remainder after the last streak is no output
+        for (int s = K - 1; s >= 0; --s) {
+          // inputs and cell of streak s
+          R pos[3] = {dx, dy, dz}, rem[3] = {dd[0], dd[1], dd[2]};
+          int cs = ci;
+          R scratch[12];
+          for (int k = 0; k < s; ++k) cs = face_neighbor(a.g, cs, streak(pos, rem, q, scratch));
+          load_lacc(a, interior_to_voxel(a.g, cs), lacc);
+          streak_bwd(pos, rem, q, lacc, lpos, lrem);
+        }
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      a.lin[(size_t)k * n + t] = ld[k];
-      a.lin[(size_t)(3 + k) * n + t] = lu[k];
+        for (int k = 0; k < 3; ++k) {
+          ld[k] = lpos[k];
+          ldd[k] = lrem[k];
+        }
+      }
+      displacement_bwd(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, ldd, lu);
+      boris_bwd(e0, e1, e2, b0, b1, a.qdt_2mc, dx, dy, dz, ux0, uy0, uz0, lu, ld, lf);
+      double* li = a.lin + (size_t)t * kAdjRec;
+      typedef double D4 __attribute__((ext_vector_type(4)));
+      D4 r0, r1;
+      r0.x = ld[0]; r0.y = ld[1]; r0.z = ld[2]; r0.w = lu[0];
+      r1.x = lu[1]; r1.y = lu[2]; r1.z = 0;     r1.w = 0;
+      reinterpret_cast<D4*>(li)[0] = r0;
+      reinterpret_cast<D4*>(li)[1] = r1;
     }
   }
-  double* li = a.linterp + (size_t)v * kInterpStride;
-#pragma unroll
-  for (int k = 0; k < 18; ++k) li[k] = lf[k];
+  add_interp_runs(a.linterp, key, lf);
 }
 
 // ---------------------------------------------------------------------------
