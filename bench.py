@@ -295,6 +295,10 @@ def main():
                          "(0 = skip)")
     ap.add_argument("--column-gradient-steps", type=int, default=1000,
                     help="steps of the electron column's dA_3/dB_z adjoint run (0 = skip)")
+    ap.add_argument("--precision-steps", type=int, default=2500,
+                    help="steps of the FP32 / compensated FP32 / FP64 Weibel runs whose "
+                         "B-energy growth curves are compared, to past the saturation "
+                         "(0 = skip)")
     ap.add_argument("--column-steps", type=int, default=20,
                     help="timed FP32 steps of the 192x192x64 x 32 ppc electron-column "
                          "configuration, reported beside the metric (0 = skip)")
@@ -414,6 +418,32 @@ def main():
                      column_gradient_forward_ms_per_step=round(cf_ms, 4),
                      column_gradient_backward_ms_per_step=round(cb_ms, 4),
                      column_gradient_hbm_bytes=mem)
+
+    if args.precision_steps > 0:
+        from cabanapic_amd import precision
+        r = precision.weibel_report(args.precision_steps, n=args.n, nppc=args.nppc)
+        keys = ("window_steps", "curve_max_abs_log_err", "curve_first_steps",
+                "curve_max_abs_log_err_first", "curve_max_abs_log_err_window",
+                "curve_max_rel_err_window", "growth_rate", "growth_rate_rel_err", "saturation",
+                "saturation_rel_err", "saturation_step", "ms_per_step")
+        rep = {name: {k: r[name][k] for k in keys} for name in ("fp32", "fp32_compensated")}
+        rep["fp64"] = {k: r["fp32"][k + "_fp64"] for k in ("growth_rate", "saturation",
+                                                             "saturation_step")}
+        rep["fp64"]["ms_per_step"] = r["fp64_ms_per_step"]
+        for name in ("fp32", "fp32_compensated"):
+            f = rep[name]
+            print(f"# {args.precision_steps} Weibel steps, {name} vs fp64: growth rate "
+                  f"{f['growth_rate']:.5f} ({f['growth_rate_rel_err']:.2e} rel.), saturation "
+                  f"{f['saturation']:.4e} at step {f['saturation_step']} "
+                  f"({f['saturation_rel_err']:.2e} rel.), curve |ln ratio| max "
+                  f"{f['curve_max_abs_log_err']:.2e} ({f['curve_max_abs_log_err_first']:.2e} "
+                  f"over the first {f['curve_first_steps']} steps, "
+                  f"{f['curve_max_abs_log_err_window']:.2e} in the growth window); "
+                  f"{f['ms_per_step']:.3f} ms/step", file=sys.stderr,
+                  flush=True)
+        extra.update(precision_weibel_steps=args.precision_steps, precision_weibel=rep,
+                     compensated_over_plain_step=round(rep["fp32_compensated"]["ms_per_step"] /
+                                                       rep["fp32"]["ms_per_step"], 3))
 
     if args.column_steps > 0:
         dc = decks.dioctron_3d(np.float32)

@@ -94,6 +94,7 @@ SIGNATURES = {
     "cpic_advance_e": (ctypes.c_int, [_P, _D, _D, _D, _D]),
     "cpic_update_ghosts_j": (ctypes.c_int, [_P]),
     "cpic_set_field_solver": (ctypes.c_int, [_P, _I32]),
+    "cpic_set_compensated_deposit": (ctypes.c_int, [_P, _I32]),
     "cpic_energies": (ctypes.c_int, [_P, ctypes.POINTER(_D), ctypes.POINTER(_D)]),
     "cpic_kinetic_energy": (ctypes.c_int, [_P, _D, _D, ctypes.POINTER(_D)]),
     "cpic_sort_particles": (ctypes.c_int, [_P]),

@@ -138,6 +138,7 @@ struct cpic_context : ColumnSet {
   hipStream_t stream = nullptr;
   int device = 0;
   int solver = CPIC_FIELD_SOLVER_EM;  // field solver of advance_b/e and energies
+  bool compensated = false;           // compensated lane accumulators in the push
 
   // stage timing: bit s set = stage s bracketed by events
   unsigned timing = 0;
@@ -307,9 +308,11 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   // 32-bit byte offsets; columns too deep for that are packed at every push.
   const bool group_fits = (int64_t)c->D * 64 * kSlotLen * (int64_t)sizeof(R) < INT32_MAX;
   if (compact || !group_fits)
-    k_push_pass<R><<<blocks, kBlock, 0, c->stream>>>(p);
+    c->compensated ? k_push_pass<R, true><<<blocks, kBlock, 0, c->stream>>>(p)
+                   : k_push_pass<R, false><<<blocks, kBlock, 0, c->stream>>>(p);
   else
-    k_push_rows<R><<<blocks, kBlock, 0, c->stream>>>(p);
+    c->compensated ? k_push_rows<R, true><<<blocks, kBlock, 0, c->stream>>>(p)
+                   : k_push_rows<R, false><<<blocks, kBlock, 0, c->stream>>>(p);
   CHECK_LAUNCH();
 
   // the streak after the 1st crossing, found through the departure masks
@@ -327,7 +330,8 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   ar.cdt_dz = cdt_dz;
   ar.qsp = qsp;
   ar.err = c->err;
-  k_arrive_pass<R><<<blocks, kBlock, 0, c->stream>>>(ar);
+  c->compensated ? k_arrive_pass<R, true><<<blocks, kBlock, 0, c->stream>>>(ar)
+                 : k_arrive_pass<R, false><<<blocks, kBlock, 0, c->stream>>>(ar);
   CHECK_LAUNCH();
 
   // the streaks after the 2nd and 3rd crossings, from the record lists
@@ -345,7 +349,8 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
     f.qsp = qsp;
     f.last = (pass == 1) ? 1 : 0;
     f.err = c->err;
-    k_finish_pass<R><<<blocks, kBlock, 0, c->stream>>>(f);
+    c->compensated ? k_finish_pass<R, true><<<blocks, kBlock, 0, c->stream>>>(f)
+                   : k_finish_pass<R, false><<<blocks, kBlock, 0, c->stream>>>(f);
     CHECK_LAUNCH();
   }
   SpillPass<R> sp{cs, ovf, (R*)c->acc, c->cg, (int)c->ncells, qsp, c->err};
@@ -1178,6 +1183,15 @@ int cpic_set_field_solver(cpic_context* c, int32_t solver) {
     if (solver == CPIC_FIELD_SOLVER_ES_1D && (c->ny > 1 || c->nz > 1))
       return fail(CPIC_ERR_UNSUPPORTED, "the ES field solver supports 1D (ny = nz = 1) only");
     c->solver = solver;
+    return CPIC_OK;
+  });
+}
+
+int cpic_set_compensated_deposit(cpic_context* c, int32_t enabled) {
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (enabled != 0 && enabled != 1) return fail(CPIC_ERR_INVALID, "enabled must be 0 or 1");
+    c->compensated = enabled == 1;
     return CPIC_OK;
   });
 }

@@ -228,6 +228,47 @@ __device__ __forceinline__ void streak_values(R q, R dx, R dy, R dz, R mx, R my,
   quadrants(q, dz, mx, my, v5, o + 8);  // jz: (z,x,y)
 }
 
+// The 12 accumulator values of a lane's cell.  Plain: one real_t sum each,
+// added to in streak order.  Compensated (K = true): each sum carries the
+// rounding error of its additions, recovered exactly by TwoSum (Knuth; six
+// operations, no branch) and added back when the sum is read; the result is
+// the sum rounded once or twice instead of once per addition.
+template <class R, bool K> struct LaneAcc {
+  R s[12];
+  R c[K ? 12 : 1];
+  __device__ __forceinline__ void zero() {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) s[k] = 0;
+    if (K) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) c[K ? k : 0] = 0;
+    }
+  }
+  __device__ __forceinline__ void add(int k, R v) {
+    if (K) {
+      const R t = s[k] + v;
+      const R vv = t - s[k];
+      const R e = (s[k] - (t - vv)) + (v - vv);
+      c[K ? k : 0] += e;
+      s[k] = t;
+    } else {
+      s[k] += v;
+    }
+  }
+  __device__ __forceinline__ R total(int k) const { return K ? s[k] + c[K ? k : 0] : s[k]; }
+};
+
+template <class R>
+__device__ __forceinline__ void add12(R acc[12], const R o[12]) {
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc[k] += o[k];
+}
+template <class R, bool K>
+__device__ __forceinline__ void add12(LaneAcc<R, K>& acc, const R o[12]) {
+#pragma unroll
+  for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
+}
+
 // One iteration of the move_p loop body (move_p.h:101-209, 218-227,
 // 368-370) for a particle in the calling lane's cell: the streak up to the
 // first face or the end of the track.  Adds the streak's 12 contributions to
@@ -235,8 +276,8 @@ __device__ __forceinline__ void streak_values(R q, R dx, R dy, R dz, R mx, R my,
 // +x,+y,+z) or -1 when the track ends in the cell.  On a crossing pos is
 // already expressed in the neighbour's coordinates (the crossed coordinate
 // flipped to the entry face).
-template <class R>
-__device__ __forceinline__ int streak(R pos[3], R rem[3], R q, R acc[12]) {
+template <class R, class A>
+__device__ __forceinline__ int streak(R pos[3], R rem[3], R q, A& acc) {
   R dir[3], len3[3], dsp[3], mid[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
@@ -259,8 +300,7 @@ __device__ __forceinline__ int streak(R pos[3], R rem[3], R q, R acc[12]) {
   const R v5 = streak_v5(q, dsp[0], dsp[1], dsp[2]);
   R o[12];
   streak_values(q, dsp[0], dsp[1], dsp[2], mid[0], mid[1], mid[2], v5, o);
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] += o[k];
+  add12(acc, o);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     rem[k] -= dsp[k];
@@ -452,7 +492,7 @@ template <class R> struct PushPass {
 // displacement and the first streak from these in pass 2.  Its row is
 // marked in the occupancy mask (reserved until pass 2 has read it) and in
 // the departure mask of face d.
-template <class R>
+template <class R, bool K>
 __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // an earlier push failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
@@ -473,9 +513,8 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   const R one = 1.;
   const R one_third = 1. / 3.;
   const R qdt_2mc = a.qdt_2mc;
-  R acc[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = 0;
+  LaneAcc<R, K> acc;
+  acc.zero();
   int ns = 0, nmov = 0;
   unsigned long long m_in = 0;
   const size_t MP = a.cs.nc_pad;
@@ -513,7 +552,7 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
         R o[12];
         streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) acc[k] += o[k];
+        for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
         px = nx;
         py = ny;
         pz = nz;
@@ -584,10 +623,10 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   if (valid) {
     if (a.accumulate) {
 #pragma unroll
-      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc[k];
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc.total(k);
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc[k];
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc.total(k);
     }
   }
   // the remaining mask and departure words of the column
@@ -626,7 +665,7 @@ __device__ __forceinline__ int sel6(int d, int a0, int a1, int a2, int a3, int a
 // Passes 2-4: the next streak of every particle that entered the lane's cell.
 // Each lane walks its own arrivals (all directions, in direction then record
 // order) in one loop, so a wave iterates max over lanes of the lane totals.
-template <class R>
+template <class R, bool K>
 __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // this push already failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
@@ -661,9 +700,8 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
       asm volatile("" : "+v"(w0));  // wait for the word here, before the loop
       rows.start(l, w0);
     }
-    R acc[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) acc[k] = 0;
+    LaneAcc<R, K> acc;
+    acc.zero();
     typedef typename Vec4T<R>::type V4;
     // (d, k): the next record, walked in direction then record order.  Every
     // lane issues every load of a batch (lanes past their last record read
@@ -723,7 +761,7 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
       if (total > 0) {
         const size_t N = a.ncells;
 #pragma unroll
-        for (int kq = 0; kq < 12; ++kq) a.acc[kq * N + v] += acc[kq];
+        for (int kq = 0; kq < 12; ++kq) a.acc[kq * N + v] += acc.total(kq);
       }
     }
   }
@@ -742,7 +780,7 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
 // The loads of row j + 1 are issued before the stores of row j, and with a
 // fixed number of instructions in between, the wait for them does not wait
 // for those stores.
-template <class R>
+template <class R, bool K>
 __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
   if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // an earlier push failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
@@ -767,9 +805,8 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
   const R one = 1.;
   const R one_third = 1. / 3.;
   const R qdt_2mc = a.qdt_2mc;
-  R acc[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = 0;
+  LaneAcc<R, K> acc;
+  acc.zero();
   int hi = 0, nmov = 0;
   unsigned long long m_in = 0, m_out = 0;
   unsigned long long dw0 = 0, dw1 = 0, dw2 = 0, dw3 = 0, dw4 = 0, dw5 = 0;
@@ -824,7 +861,7 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
         R o[12];
         streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
 #pragma unroll
-        for (int k = 0; k < 12; ++k) acc[k] += o[k];
+        for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
         px = nx;
         py = ny;
         pz = nz;
@@ -893,10 +930,10 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
     const size_t N = a.ncells;
     if (a.accumulate) {
 #pragma unroll
-      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc[k];
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc.total(k);
     } else {
 #pragma unroll
-      for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc[k];
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc.total(k);
     }
   }
   for (int w = (lmax + 63) >> 6; w < MW; ++w) {
@@ -931,7 +968,7 @@ template <class R> struct ArrivePass {
 // column, recomputes its displacement and first streak (the same operations
 // as pass 1, without the deposit, which pass 1 made), and runs the next
 // streak.  Afterwards each cell releases the rows of its own departures.
-template <class R>
+template <class R, bool K>
 __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
   if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // this push already failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;  // < nc_pad
@@ -959,9 +996,8 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
     settle(w0);
     rows.start(l, w0);
   }
-  R acc[12];
-#pragma unroll
-  for (int k = 0; k < 12; ++k) acc[k] = 0;
+  LaneAcc<R, K> acc;
+  acc.zero();
   int kd[6] = {0, 0, 0, 0, 0, 0};
   int total_all = 0;
   const R* dummy = a.cs.out;
@@ -1058,7 +1094,7 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
     if (total_all > 0) {
       const size_t N = a.ncells;
 #pragma unroll
-      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc[k];
+      for (int k = 0; k < 12; ++k) a.acc[k * N + v] += acc.total(k);
     }
     // release the rows of this cell's own departures: only their mask bits
     // are cleared here.  Other lanes may still be reading those rows in this

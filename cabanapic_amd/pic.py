@@ -199,6 +199,11 @@ class Simulation:
         code = {"em": _capi.FIELD_SOLVER_EM, "es_1d": _capi.FIELD_SOLVER_ES_1D}[solver]
         check(self.lib.cpic_set_field_solver(self._h, code))
 
+    def set_compensated_deposit(self, enabled):
+        """Compensated (TwoSum) summation of each cell's current sums inside
+        the push passes; off by default."""
+        check(self.lib.cpic_set_compensated_deposit(self._h, 1 if enabled else 0))
+
     def update_ghosts_j(self):
         check(self.lib.cpic_update_ghosts_j(self._h))
 

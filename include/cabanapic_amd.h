@@ -178,6 +178,16 @@ int cpic_advance_e(cpic_context *ctx, double px, double py, double pz,
 #define CPIC_FIELD_SOLVER_ES_1D 1
 int cpic_set_field_solver(cpic_context *ctx, int32_t solver);
 
+/* Compensated summation of the current deposit (no counterpart in the
+ * reference, whose push adds every streak's contributions to the
+ * accumulator in real_t, src/push.h:236-254 and src/move_p.h:175-190).
+ * enabled = 1: each cell's 12 accumulator sums inside a push pass carry
+ * their rounding error (TwoSum) and add it back when the pass writes them;
+ * 0 (default): plain real_t sums.  Particles are unaffected; the
+ * accumulators differ from the plain push's by rounding only.  Meant for
+ * FP32 runs whose field energies are compared with FP64. */
+int cpic_set_compensated_deposit(cpic_context *ctx, int32_t enabled);
+
 /* serial_update_ghosts on jfx,jfy,jfz (src/fields.h:102-271) followed by
  * serial_update_ghosts_B on J (src/fields.h:11-100): the ghost update that
  * advance_e performs first, exposed on its own. */

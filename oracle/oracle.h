@@ -50,6 +50,11 @@ extern "C" {
                        double *abs_acc, R qdt_2mc, R cdt_dx, R cdt_dy,         \
                        R cdt_dz, R qsp, long nx, long ny, long nz,             \
                        int boundary);                                          \
+  long ora_push_sum##S(R *x, R *y, R *z, R *ux, R *uy, R *uz, const R *w,      \
+                       int32_t *cell, long np, const R *interp, R *acc,        \
+                       double *abs_acc, double *sum_acc, R qdt_2mc, R cdt_dx,  \
+                       R cdt_dy, R cdt_dz, R qsp, long nx, long ny, long nz,   \
+                       int boundary);                                          \
   long ora_push_omp##S(R *x, R *y, R *z, R *ux, R *uy, R *uz, const R *w,      \
                        int32_t *cell, long np, const R *interp, R *acc,        \
                        R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp,         \

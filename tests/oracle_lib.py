@@ -60,6 +60,8 @@ def _declare(L):
         g("ora_push").restype = Lg
         g("ora_push_abs").argtypes = [P] * 8 + [Lg, P, P, P] + [R] * 5 + [Lg, Lg, Lg, I]
         g("ora_push_abs").restype = Lg
+        g("ora_push_sum").argtypes = [P] * 8 + [Lg, P, P, P, P] + [R] * 5 + [Lg, Lg, Lg, I]
+        g("ora_push_sum").restype = Lg
         g("ora_push_omp").argtypes = [P] * 8 + [Lg, P, P] + [R] * 5 + [Lg, Lg, Lg, I, I]
         g("ora_push_omp").restype = Lg
         g("ora_unload_accumulator").argtypes = [P, P, Lg, Lg, Lg, R, R, R, R]
@@ -132,7 +134,15 @@ class OracleState:
     def clear_accumulator(self):
         self.fn("ora_clear_accumulator")(_p(self.acc), self.ng)
 
-    def push(self, qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, boundary=1, nthreads=1, abs_acc=None):
+    def push(self, qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp, boundary=1, nthreads=1, abs_acc=None,
+             sum_acc=None):
+        if sum_acc is not None:
+            for a in (abs_acc, sum_acc):
+                assert a is None or (a.dtype == np.float64 and a.size == 12 * self.ng)
+            return self.fn("ora_push_sum")(*self._pp(), self.np, _p(self.interp), _p(self.acc),
+                                           None if abs_acc is None else _p(abs_acc),
+                                           _p(sum_acc), qdt_2mc, cdt_dx, cdt_dy, cdt_dz, qsp,
+                                           self.nx, self.ny, self.nz, boundary)
         if abs_acc is not None:
             assert abs_acc.dtype == np.float64 and abs_acc.size == 12 * self.ng
             return self.fn("ora_push_abs")(*self._pp(), self.np, _p(self.interp), _p(self.acc),

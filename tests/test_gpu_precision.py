@@ -1,0 +1,104 @@
+"""FP32 precision of the push's current deposit, and FP32 against FP64 on the
+full Weibel configuration.
+
+* The compensated deposit (cpic_set_compensated_deposit): particles stay
+  bit-identical to the oracle's, and each accumulator slot equals the exact
+  sum of the oracle's own FP32 contributions (summed by the oracle in FP64)
+  to a few roundings of the total, where the plain FP32 sum is off by up to
+  one rounding per addition.
+* 2500 steps of the Weibel deck (128^3 cells x 32 particles per cell, past
+  the saturation near step 2100) in FP64, FP32 and compensated FP32 from the
+  same initial plasma: the B-energy growth curve of both FP32 runs against
+  FP64 (its error over the first 500 steps, the configuration's length, and
+  over the growth window; the growth rate; the saturation level and step).
+"""
+import json
+
+import numpy as np
+import pytest
+
+import states
+from cabanapic_amd import precision
+from test_gpu_parity import assert_particles_bitwise, sim_from_state
+
+pytestmark = pytest.mark.gpu
+
+
+def _push_twice(st, k, compensated):
+    """Two pushes (the first after the upload packs the columns, the second
+    runs in place); particles and accumulators after each."""
+    out = []
+    with sim_from_state(st) as sim:
+        sim.set_compensated_deposit(compensated)
+        sim.load_interpolator_array()
+        for _ in range(2):
+            sim.clear_accumulator_array()
+            sim.push(**k)
+            out.append((sim.download_particles(),
+                        sim.download_accumulators().astype(np.float64)))
+        assert sim.device_errors() == 0
+    return out
+
+
+@pytest.mark.parametrize("push", [0, 1], ids=["packing_push", "in_place_push"])
+def test_compensated_deposit_against_exact_sum(push):
+    # long sums: 4^3 cells x 400 particles per cell, about 400 additions per
+    # slot, a third of the particles crossing
+    st, k = states.random_state(4, 4, 4, 400, np.float32, seed=71, umax=0.9)
+    st.load_interpolator()
+    got_p, acc_p = _push_twice(st, k, False)[push]
+    got_c, acc_c = _push_twice(st, k, True)[push]
+    absacc = np.zeros(st.acc.size)
+    exact = np.zeros(st.acc.size)
+    for i in range(push + 1):
+        st.clear_accumulator()
+        if i == push:
+            st.push(**k, abs_acc=absacc, sum_acc=exact)
+        else:
+            st.push(**k)
+    assert_particles_bitwise(got_p, st)
+    assert_particles_bitwise(got_c, st)
+    absacc = absacc.reshape(acc_p.shape)
+    exact = exact.reshape(acc_p.shape)
+    eps = float(np.finfo(np.float32).eps)
+    tiny = float(np.finfo(np.float32).tiny)
+    err_p = np.abs(acc_p - exact)
+    err_c = np.abs(acc_c - exact)
+    # both within the reordered-sum bound of the parity tests
+    assert (err_p <= 64 * eps * absacc + tiny).all()
+    assert (err_c <= 64 * eps * absacc + tiny).all()
+    # compensated: a few roundings of the pass totals (each of the push's
+    # passes adds its compensated total to the slot with one rounding), so
+    # in roundings of the exact sum the typical error is below one, and the
+    # total error far below the plain sums'
+    nz = np.abs(exact) > 0
+    ulps_c = err_c[nz] / (eps * np.abs(exact[nz]))
+    ulps_p = err_p[nz] / (eps * np.abs(exact[nz]))
+    mp, mc = float(err_p.sum()), float(err_c.sum())
+    print(f"|acc - exact| over {acc_p.size} slots: sum plain {mp:.3e}, compensated {mc:.3e}; "
+          f"median in eps|exact|: plain {np.median(ulps_p):.2f}, compensated "
+          f"{np.median(ulps_c):.2f}")
+    assert np.median(ulps_c) <= 1.0
+    assert mc < 0.25 * mp
+
+
+@pytest.mark.timeout(900)
+def test_weibel_fp32_against_fp64_to_saturation():
+    r = precision.weibel_report(steps=2500, first=500)
+    figures = {k: r[k] for k in ("steps", "deck", "dt", "fp64_ms_per_step", "fp32",
+                                 "fp32_compensated")}
+    print(json.dumps(figures, indent=1))
+    b64 = r["history"]["fp64"][:, 1]
+    lo, hi = precision.growth_window(b64)
+    # the instability grows by over three orders of magnitude from the noise
+    # level (the window holds the middle half of that range in ln W_B)
+    assert b64.max() > 1000 * b64[:10].mean()
+    assert b64[hi - 1] > 30 * b64[lo]
+    assert hi < int(np.argmax(b64)) < 2500 - 100  # saturates inside the run
+    for name in ("fp32", "fp32_compensated"):
+        f = r[name]
+        assert f["curve_max_abs_log_err_first"] < 1e-3, (name, f)
+        assert f["curve_max_abs_log_err_window"] < 1e-3, (name, f)
+        assert f["growth_rate_rel_err"] < 1e-3, (name, f)
+        assert f["saturation_rel_err"] < 1e-3, (name, f)
+        assert abs(f["saturation_step"] - f["saturation_step_fp64"]) <= 10, (name, f)

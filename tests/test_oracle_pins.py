@@ -14,6 +14,11 @@ checks leave open, and proof that the pins bite.
   threads reorders each slot's terms, nothing else); a second call on fresh
   accumulators gives the same result (the duplicates are reset); ora_run
   with threads agrees with one thread over 50 steps of the 2stream-em deck.
+* ora_push_sum's signed sink (the exact-sum reference of the compensated
+  device deposit): in the FP64 build it equals the accumulators bit for bit
+  (same terms added in the same order in the same type); in the FP32 build,
+  with one non-crossing particle per cell (one term per slot) it equals them
+  exactly, and on a random state it is within the reordered-sum bound.
 
 The planted-mistake tests build the oracle with an ORA_PLANT_* switch
 (a dropped factor, a dropped term, a thread's duplicate left out of the sum,
@@ -171,3 +176,37 @@ def test_omp_pins_catch_planted_mistake(plant, plant_dir):
     L = oracle_lib.build_planted(plant, plant_dir)
     assert omp_push_failures(L, F32, threads=(2, 7)), f"{plant} passed the push pins"
     assert omp_run_failures(L, F64, nsteps=5, threads=(2,)), f"{plant} passed the run pins"
+
+
+# ----------------------------------------------------------------------------
+# the signed-sum sink of the push
+
+def _push_sum(st, k):
+    st.load_interpolator()
+    st.clear_accumulator()
+    a = np.zeros(st.acc.size)
+    sm = np.zeros(st.acc.size)
+    moved = st.push(**k, abs_acc=a, sum_acc=sm)
+    return moved, a.reshape(st.acc.shape), sm.reshape(st.acc.shape)
+
+
+def test_push_sum_fp64_equals_accumulators():
+    st, k = states.random_state(4, 3, 5, 40, F64, seed=81, umax=0.9)
+    moved, _, sm = _push_sum(st, k)
+    assert moved > 0
+    assert np.array_equal(sm, st.acc)
+
+
+def test_push_sum_fp32_single_term_and_bound():
+    st, k = states.random_state(5, 4, 3, 1, F32, seed=82, umax=0.05)
+    for m in ("x", "y", "z"):
+        st.p[m][:] *= F32(0.5)  # away from the faces: no crossing
+    moved, _, sm = _push_sum(st, k)
+    assert moved == 0  # one streak per cell: one term per slot
+    assert np.array_equal(sm, st.acc.astype(np.float64))
+    st, k = states.random_state(4, 4, 4, 100, F32, seed=83, umax=0.9)
+    moved, a, sm = _push_sum(st, k)
+    assert moved > 0
+    err = np.abs(st.acc.astype(np.float64) - sm)
+    assert (err <= 64 * np.finfo(F32).eps * a + np.finfo(F32).tiny).all()
+    assert err.max() > 0  # the float sums do round
