@@ -72,6 +72,7 @@ SIGNATURES = {
     "cpic_num_particles": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "cpic_download_particle_tags": (ctypes.c_int, [_P, _I64, _P]),
     "cpic_column_rows": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+    "cpic_reserve_column_rows": (ctypes.c_int, [_P, _I64]),
     "cpic_field_energies_noalloc": (ctypes.c_int, [_P, _I32, _I64, _I64, _I64, _P, _P]),
     "cpic_adjoint_create": (ctypes.c_int, [_P, _PP]),
     "cpic_adjoint_destroy": (ctypes.c_int, [_P]),

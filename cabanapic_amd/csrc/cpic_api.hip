@@ -135,6 +135,18 @@ struct cpic_context : ColumnSet {
   unsigned long long* gauss_out = nullptr;  // [max |r| or |r - base|, max |rho / eps0|]
   bool gauss_have_base = false;
 
+  // dump snapshots (cpic_snapshot_begin / _wait): two slots, each a device
+  // staging area and its pinned host copy, filled on the copy stream
+  struct Snapshot {
+    void* dev = nullptr;
+    void* host = nullptr;
+    size_t bytes = 0;
+    hipEvent_t staged = nullptr;  // on the step stream, after the gather
+    hipEvent_t ready = nullptr;   // on the copy stream, after the download
+    bool pending = false;         // begun, not yet waited for
+  } snap[2];
+  hipStream_t copy_stream = nullptr;
+
   hipStream_t stream = nullptr;
   int device = 0;
   int solver = CPIC_FIELD_SOLVER_EM;  // field solver of advance_b/e and energies
@@ -592,10 +604,10 @@ int note_occupancy(cpic_context* c) {
   return CPIC_OK;
 }
 
-// The particles packed into columns of 1.5x the rows, on the device.
+// The particles packed into columns of D rows (a multiple of 64, more than
+// the current rows), on the device; row order within a column is kept.
 template <class R>
-int regrow_columns(cpic_context* c) {
-  const int D = (c->D * 3 / 2 + 63) / 64 * 64;
+int regrow_columns(cpic_context* c, int D) {
   ColumnSet s;
   if (int t = alloc_column_set(c, D, c->M1, c->M2, s)) return t;
   const int blocks = cdiv(c->nc_pad, kBlock);
@@ -624,7 +636,113 @@ int check_headroom(cpic_context* c) {
     return fail(CPIC_ERR_DEVICE, "an earlier push failed on the device (flags=%d)",
                 c->occ_host[1]);
   if (c->D == 0 || c->occ_host[0] < c->D - std::max(8, c->D / 4)) return CPIC_OK;
-  return with_real(c, [&](auto r) { return regrow_columns<decltype(r)>(c); });
+  const int D = (c->D * 3 / 2 + 63) / 64 * 64;  // 1.5x the rows
+  return with_real(c, [&](auto r) { return regrow_columns<decltype(r)>(c, D); });
+}
+
+}  // namespace
+
+namespace {
+
+// ---- dump snapshots ----------------------------------------------------------
+// Layout of a slot (device staging and pinned host copy alike): x[n], ux[n]
+// (reals), cell[n] (int32), ex[nx] (reals), energies[3] (double), each part
+// starting on a 256-byte boundary.
+struct SnapLayout {
+  size_t x, ux, cell, ex, energies, bytes;
+};
+
+SnapLayout snap_layout(const cpic_context* c) {
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const size_t n = (size_t)c->np, rs = c->rsz();
+  SnapLayout L;
+  L.x = 0;
+  L.ux = L.x + up(rs * n);
+  L.cell = L.ux + up(rs * n);
+  L.ex = L.cell + up(sizeof(int) * n);
+  L.energies = L.ex + up(rs * (size_t)c->nx);
+  L.bytes = L.energies + 3 * sizeof(double);
+  return L;
+}
+
+void free_snapshot(cpic_context::Snapshot& sn) {
+  if (sn.dev) (void)hipFree(sn.dev);
+  if (sn.host) (void)hipHostFree(sn.host);
+  if (sn.staged) (void)hipEventDestroy(sn.staged);
+  if (sn.ready) (void)hipEventDestroy(sn.ready);
+  sn = cpic_context::Snapshot{};
+}
+
+int snapshot_begin(cpic_context* c, int slot) {
+  if (int s = check_ctx(c)) return s;
+  if (slot != 0 && slot != 1) return fail(CPIC_ERR_INVALID, "snapshot slot must be 0 or 1");
+  auto& sn = c->snap[slot];
+  if (sn.pending)
+    return fail(CPIC_ERR_INVALID, "snapshot slot %d was begun and not waited for", slot);
+  const SnapLayout L = snap_layout(c);
+  // first use, or the particle count changed: (re)allocate this slot and the
+  // other one too unless it holds a snapshot not yet waited for
+  for (int k : {slot, 1 - slot}) {
+    auto& q = c->snap[k];
+    if (q.bytes == L.bytes || q.pending) continue;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    free_snapshot(q);
+    if (int s = alloc(&q.dev, L.bytes)) return s;
+    if (hipHostMalloc(&q.host, L.bytes, hipHostMallocDefault) != hipSuccess) {
+      q.host = nullptr;
+      free_snapshot(q);
+      return fail(CPIC_ERR_HIP, "pinned allocation of %zu bytes failed", L.bytes);
+    }
+    HIP_TRY(hipEventCreateWithFlags(&q.staged, hipEventDisableTiming));
+    HIP_TRY(hipEventCreateWithFlags(&q.ready, hipEventDisableTiming));
+    q.bytes = L.bytes;
+  }
+  if (!c->copy_stream)
+    HIP_TRY(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  char* d = (char*)sn.dev;
+  const long nthreads = std::max<long>(std::max<long>(c->nc, c->nx), 3);
+  const bool have = c->np > 0 && c->D > 0;
+  int s = with_real(c, [&](auto r) -> int {
+    using R = decltype(r);
+    k_snapshot<R><<<cdiv(nthreads, kBlock), kBlock, 0, c->stream>>>(
+        have ? (const R*)c->cols[c->cur] : nullptr, have ? c->cnt[c->cur] : nullptr,
+        have ? c->mask[c->cur] : nullptr, c->cg, c->D, (int)c->nc_pad, have ? c->np : 0,
+        (const R*)c->fields, c->grid.sy, c->grid.sz, c->red_out, (R*)(d + L.x),
+        (R*)(d + L.ux), (int*)(d + L.cell), (R*)(d + L.ex), (double*)(d + L.energies), c->err);
+    CHECK_LAUNCH();
+    return CPIC_OK;
+  });
+  if (s) return s;
+  HIP_TRY(hipEventRecord(sn.staged, c->stream));
+  HIP_TRY(hipStreamWaitEvent(c->copy_stream, sn.staged, 0));
+  HIP_TRY(hipMemcpyAsync(sn.host, sn.dev, L.bytes, hipMemcpyDeviceToHost, c->copy_stream));
+  HIP_TRY(hipEventRecord(sn.ready, c->copy_stream));
+  sn.pending = true;
+  return CPIC_OK;
+}
+
+int snapshot_wait(cpic_context* c, int slot, cpic_snapshot* out) {
+  if (int s = check_ctx(c)) return s;
+  if (slot != 0 && slot != 1) return fail(CPIC_ERR_INVALID, "snapshot slot must be 0 or 1");
+  if (!out) return fail(CPIC_ERR_INVALID, "null snapshot");
+  auto& sn = c->snap[slot];
+  if (!sn.pending) return fail(CPIC_ERR_INVALID, "no snapshot begun on slot %d", slot);
+  HIP_TRY(hipEventSynchronize(sn.ready));
+  sn.pending = false;
+  const SnapLayout L = snap_layout(c);
+  const char* h = (const char*)sn.host;
+  out->n = c->np;
+  out->nx = c->nx;
+  out->real_size = c->real_size;
+  out->x = h + L.x;
+  out->ux = h + L.ux;
+  out->cell = (const int32_t*)(h + L.cell);
+  out->ex_line = h + L.ex;
+  const double* e = (const double*)(h + L.energies);
+  out->e_energy = e[0];
+  out->b_energy = e[1];
+  out->kinetic_energy = e[2];
+  return CPIC_OK;
 }
 
 }  // namespace
@@ -750,6 +868,7 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
     if (s) return bail(s);
     if (hipMemsetAsync(c->err, 0, sizeof(int), c->stream) != hipSuccess ||
         hipMemsetAsync(c->movers, 0, sizeof(unsigned int), c->stream) != hipSuccess ||
+        hipMemsetAsync(c->red_out, 0, sizeof(double) * 4, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       return bail(fail(CPIC_ERR_HIP, "context initialisation failed"));
     *out = c;
@@ -775,6 +894,9 @@ int cpic_destroy(cpic_context* c) {
       if (p) (void)hipFree(p);
     if (c->occ_event) (void)hipEventDestroy(c->occ_event);
     if (c->occ_host) (void)hipHostFree(c->occ_host);
+    if (c->copy_stream) (void)hipStreamSynchronize(c->copy_stream);
+    for (auto& sn : c->snap) free_snapshot(sn);
+    if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return CPIC_OK;
@@ -992,6 +1114,20 @@ int cpic_download_particle_tags(cpic_context* c, int64_t n, int64_t* tag) {
   return guarded([&] {
     return download_particles(c, n, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
                               nullptr, nullptr, tag);
+  });
+}
+
+int cpic_reserve_column_rows(cpic_context* c, int64_t rows) {
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (rows < 0 || rows > (1L << 20))
+      return fail(CPIC_ERR_INVALID, "rows out of range [0, 2^20]");
+    if (c->D == 0 || rows <= c->D) return CPIC_OK;
+    const int D = (int)((rows + 63) / 64 * 64);
+    int s = with_real(c, [&](auto r) { return regrow_columns<decltype(r)>(c, D); });
+    if (s) return s;
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return check_device_errors(c);
   });
 }
 
@@ -1289,6 +1425,24 @@ int cpic_sort_particles(cpic_context* c) {
     // writes each particle into the column of the cell it ends in.
     return check_ctx(c);
   });
+}
+
+int cpic_step_async(cpic_context* c, const cpic_step_params* sp, int64_t nsteps) {
+  return guarded([&]() -> int {
+    if (int s = check_ctx(c)) return s;
+    if (!sp) return fail(CPIC_ERR_INVALID, "null step parameters");
+    if (nsteps < 0) return fail(CPIC_ERR_INVALID, "negative step count");
+    if (int s = check_periodic(sp->boundary)) return s;
+    return with_real(c, [&](auto r) { return run_steps<decltype(r)>(c, sp, nsteps, nullptr); });
+  });
+}
+
+int cpic_snapshot_begin(cpic_context* c, int32_t slot) {
+  return guarded([&] { return snapshot_begin(c, slot); });
+}
+
+int cpic_snapshot_wait(cpic_context* c, int32_t slot, cpic_snapshot* out) {
+  return guarded([&] { return snapshot_wait(c, slot, out); });
 }
 
 int cpic_step(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, double* hist) {

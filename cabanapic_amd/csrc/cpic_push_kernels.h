@@ -1389,6 +1389,43 @@ k_gather_columns(const R* __restrict__ cols, const int* __restrict__ cnt,
   }
 }
 
+// Snapshot for the per-step dumps: the in-cell offset x, the momentum ux and
+// the voxel of every particle, each written at the particle's upload index
+// (the order of the particle arrays the reference dumps, src/helpers.h:
+// 31-70); lanes below nx also copy ex at voxel (lane + 1, 1, 1) (the line of
+// src/fields.h:318-350), and lane 0 the last energies (3 doubles).  A tag
+// outside [0, n) flags kErrBadCell.
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_snapshot(const R* __restrict__ cols, const int* __restrict__ cnt,
+           const unsigned long long* __restrict__ mask, CellGrid g, int D, int nc_pad, long n,
+           const R* __restrict__ ex, int sy, int sz, const double* __restrict__ energies,
+           R* __restrict__ x_out, R* __restrict__ ux_out, int* __restrict__ cell_out,
+           R* __restrict__ ex_out, double* __restrict__ energies_out, int* __restrict__ err) {
+  typedef typename Vec4T<R>::type V4;
+  const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ci < g.nx) ex_out[ci] = ex[(ci + 1) + sy + sz];
+  if (ci < 3) energies_out[ci] = energies[ci];
+  if (ci >= g.nc) return;
+  const int len = min(cnt[ci], D);
+  const int v = interior_to_voxel(g, ci);
+  unsigned long long m = 0;
+  for (int j = 0; j < len; ++j) {
+    if ((j & 63) == 0) m = mask[(size_t)(j >> 6) * nc_pad + ci];
+    if (!((m >> (j & 63)) & 1ull)) continue;
+    const V4* s = reinterpret_cast<const V4*>(cols + col_slot(ci, j, D) * kSlotLen);
+    const V4 a = s[0], b = s[1];
+    const long long t = TagBits<R>::get(b.w);
+    if (t < 0 || t >= n) {
+      atomicOr(err, kErrBadCell);
+      continue;
+    }
+    x_out[t] = a.x;
+    ux_out[t] = a.w;
+    cell_out[t] = v;
+  }
+}
+
 // Regrowth: the occupied rows of every column, packed in row order, into
 // columns of D_out rows.
 template <class R>

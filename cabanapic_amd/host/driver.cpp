@@ -3,22 +3,33 @@
 // host through the deck's initializers, uploads it, runs the step loop through
 // the C-ABI and writes energies.txt, then calls the deck's finalizer.
 //
-// Usage: cbnpic [--stages] [--chunk N] [--dump-initial FILE]
+// Usage: cbnpic [--stages] [--chunk N] [--dumps] [--dump-initial FILE]
 //   default   the fused cpic_step, energies of every step kept on the device
 //             and written after each chunk of N steps (default 200)
 //   --stages  one C-ABI call per stage and step, in the reference's order
 //             (example/example.cpp:216-279), energies read back every step
+//   --dumps   the reference's per-step partloc and ex1d files as well: the
+//             steps are queued without waiting, each followed by a snapshot
+//             into one of two slots, and a writer thread formats step k
+//             while the following steps run
 //   --dump-initial FILE
 //             write the derived constants and the initial particles to FILE
 //             and exit without touching the GPU (checks of the deck layer)
 //   --dump-constants FILE
 //             the same without the particles
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <functional>
 #include <iostream>
+#include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/cabanapic_amd.h"
@@ -59,31 +70,100 @@ void dump_energy_line(int step, real_t time, real_t e, real_t b) {
 #endif
 }
 
+// Workers that format parts of one dump line together: run(f) calls f(i)
+// for i = 0..size()-1, f(0) on the caller, and returns when all are done.
+class LinePool {
+ public:
+  explicit LinePool(unsigned n) {
+    for (unsigned i = 1; i < n; ++i) th_.emplace_back([this, i] { loop(i); });
+  }
+  ~LinePool() {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto& t : th_) t.join();
+  }
+  unsigned size() const { return (unsigned)th_.size() + 1; }
+  void run(const std::function<void(unsigned)>& f) {
+    {
+      std::lock_guard<std::mutex> lk(m_);
+      job_ = &f;
+      ++gen_;
+      left_ = (unsigned)th_.size();
+    }
+    cv_.notify_all();
+    f(0);
+    std::unique_lock<std::mutex> lk(m_);
+    done_.wait(lk, [&] { return left_ == 0; });
+  }
+
+ private:
+  void loop(unsigned i) {
+    unsigned long seen = 0;
+    for (;;) {
+      const std::function<void(unsigned)>* f;
+      {
+        std::unique_lock<std::mutex> lk(m_);
+        cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+        if (stop_) return;
+        seen = gen_;
+        f = job_;
+      }
+      (*f)(i);
+      std::lock_guard<std::mutex> lk(m_);
+      if (--left_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex m_;
+  std::condition_variable cv_, done_;
+  const std::function<void(unsigned)>* job_ = nullptr;
+  unsigned long gen_ = 0;
+  unsigned left_ = 0;
+  bool stop_ = false;
+};
+
 // dump_particles, src/helpers.h:31-70: the physical x position and ux of
-// every particle, "%e  %e " each, then a newline
+// every particle, "%e  %e " each, then a newline.  With a pool, the parts of
+// the line are formatted in parallel and written in order.
 void write_particles(FILE* fp, size_t n, const real_t* px, const real_t* ux,
                      const int32_t* cell, real_t xmin, real_t dx, size_t nx, size_t ny,
-                     size_t ng) {
-  for (size_t p = 0; p < n; ++p) {
-    size_t ix, iy, iz;
-    const int ii = cell[p];
-    RANK_TO_INDEX(ii, ix, iy, iz, nx + 2 * ng, ny + 2 * ng);
-    (void)iy;
-    (void)iz;
-    const real_t x = xmin + (ix - 1 + (px[p] + 1.0) * 0.5) * dx;
-    const real_t v = ux[p];
-    std::fprintf(fp, "%e  %e ", (double)x, (double)v);
-  }
+                     size_t ng, LinePool* pool = nullptr) {
+  const unsigned parts = pool && n >= 2048 ? pool->size() : 1;
+  std::vector<std::string> text(parts);
+  auto format = [&](unsigned t) {
+    const size_t lo = n * t / parts, hi = n * (t + 1) / parts;
+    std::string& out = text[t];
+    out.reserve((hi - lo) * 30);
+    char buf[80];
+    for (size_t p = lo; p < hi; ++p) {
+      size_t ix, iy, iz;
+      const int ii = cell[p];
+      RANK_TO_INDEX(ii, ix, iy, iz, nx + 2 * ng, ny + 2 * ng);
+      (void)iy;
+      (void)iz;
+      const real_t x = xmin + (ix - 1 + (px[p] + 1.0) * 0.5) * dx;
+      const real_t v = ux[p];
+      const int k = std::snprintf(buf, sizeof buf, "%e  %e ", (double)x, (double)v);
+      out.append(buf, (size_t)k);
+    }
+  };
+  if (parts > 1)
+    pool->run(format);
+  else
+    format(0);
+  for (const auto& t : text) std::fwrite(t.data(), 1, t.size(), fp);
   std::fprintf(fp, "\n");
 }
 
-// Field_Solver::dump_fields, src/fields.h:318-350: ex along the line y = z = 1
-void write_fields(FILE* fp, const real_t* ex, real_t xmin, real_t dx, size_t nx, size_t ny,
-                  size_t nz, size_t ng) {
+// Field_Solver::dump_fields, src/fields.h:318-350: ex along the line y = z = 1;
+// ex_line[i - 1] is ex at voxel (i, 1, 1)
+void write_fields(FILE* fp, const real_t* ex_line, real_t xmin, real_t dx, size_t nx) {
   for (size_t i = 1; i < nx + 1; i++) {
     const real_t x = xmin + (i - 0.5) * dx;
-    const size_t ii = VOXEL(i, 1, 1, nx, ny, nz, ng);
-    std::fprintf(fp, "%e %e\n", (double)x, (double)ex[ii]);
+    std::fprintf(fp, "%e %e\n", (double)x, (double)ex_line[i - 1]);
   }
   std::fprintf(fp, "\n\n");
 }
@@ -202,36 +282,25 @@ int main(int argc, char** argv) {
   // 273-277): partloc (particles) and ex1d (ex along x), with --dumps
   FILE* fptr = nullptr;
   FILE* fpfd = nullptr;
-  std::vector<real_t> dump_p[7];
-  std::vector<int32_t> dump_cell;
-  std::vector<real_t> dump_f;
+  std::unique_ptr<LinePool> pool;
   if (dumps) {
+    pool.reset(new LinePool(std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2))));
     fptr = std::fopen("partloc", "w");
     fpfd = std::fopen("ex1d", "w");
     if (!fptr || !fpfd) {
       std::fprintf(stderr, "cbnpic: cannot open partloc / ex1d\n");
       return 1;
     }
-    std::fprintf(fptr, "#step=0\n0 ");
-    write_particles(fptr, num_particles, particles.data(PositionX), particles.data(VelocityX),
-                    particles.cells(), 0, dx, nx, ny, num_ghosts);
-    chunk = 1;
-    for (auto& v : dump_p) v.resize(num_particles);
-    dump_cell.resize(num_particles);
-    dump_f.resize(9 * (size_t)(nx + 2) * (ny + 2) * (nz + 2));
   }
-  auto dump_step = [&](cpic_context* ctx, int step) {
-    if (!dumps) return;
-    CPIC_CALL(cpic_download_fields(ctx, dump_f.data()));
+  // one step's dump from a snapshot (energies line included when asked)
+  auto write_dump = [&](const cpic_snapshot& sn, int step, bool energies) {
+    if (energies)
+      dump_energy_line(step, step * dt, (real_t)sn.e_energy, (real_t)sn.b_energy);
     std::fprintf(fpfd, "#step=%d\n", step);
-    write_fields(fpfd, dump_f.data(), 0, dx, nx, ny, nz, num_ghosts);
-    CPIC_CALL(cpic_download_particles(ctx, (int64_t)num_particles, dump_p[0].data(),
-                                      dump_p[1].data(), dump_p[2].data(), dump_p[3].data(),
-                                      dump_p[4].data(), dump_p[5].data(), dump_p[6].data(),
-                                      dump_cell.data()));
+    write_fields(fpfd, (const real_t*)sn.ex_line, 0, dx, nx);
     std::fprintf(fptr, "#step=%d\n%e ", step, (double)(step * dt));
-    write_particles(fptr, num_particles, dump_p[0].data(), dump_p[3].data(), dump_cell.data(), 0,
-                    dx, nx, ny, num_ghosts);
+    write_particles(fptr, (size_t)sn.n, (const real_t*)sn.x, (const real_t*)sn.ux, sn.cell, 0,
+                    dx, nx, ny, num_ghosts, pool.get());
   };
 
   cpic_config cfg{};
@@ -254,6 +323,19 @@ int main(int argc, char** argv) {
                                   particles.cells()));
   CPIC_CALL(cpic_upload_fields(ctx, fields.data()));
 
+  // the step-0 particle line of the dumps (example/example.cpp:130-132), taken
+  // before the uncentering from a snapshot of the uploaded particles (upload
+  // order is the order of the initial arrays); it also sets up both snapshot
+  // slots before the steps
+  if (dumps) {
+    cpic_snapshot sn{};
+    CPIC_CALL(cpic_snapshot_begin(ctx, 0));
+    CPIC_CALL(cpic_snapshot_wait(ctx, 0, &sn));
+    std::fprintf(fptr, "#step=0\n0 ");
+    write_particles(fptr, (size_t)sn.n, (const real_t*)sn.x, (const real_t*)sn.ux, sn.cell, 0, dx,
+                    nx, ny, num_ghosts, pool.get());
+  }
+
   // example/example.cpp:204-213
   if (deck.perform_uncenter) {
     CPIC_CALL(cpic_load_interpolator_array(ctx));
@@ -265,6 +347,7 @@ int main(int argc, char** argv) {
   double gauss[2] = {0, 0};
   CPIC_CALL(cpic_gauss_residual(ctx, dx, dy, dz, eps0, qsp, CPIC_GAUSS_BASELINE, gauss, nullptr));
 
+  const auto loop_start = std::chrono::steady_clock::now();
   if (stages) {
     for (int step = 1; step <= num_steps; step++) {
       CPIC_CALL(cpic_load_interpolator_array(ctx));
@@ -277,7 +360,12 @@ int main(int argc, char** argv) {
       double e = 0, b = 0;
       CPIC_CALL(cpic_energies(ctx, &e, &b));
       dump_energy_line(step, step * dt, (real_t)e, (real_t)b);
-      dump_step(ctx, step);
+      if (dumps) {
+        cpic_snapshot sn{};
+        CPIC_CALL(cpic_snapshot_begin(ctx, 0));
+        CPIC_CALL(cpic_snapshot_wait(ctx, 0, &sn));
+        write_dump(sn, step, false);
+      }
     }
   } else {
     cpic_step_params sp{};
@@ -300,18 +388,79 @@ int main(int argc, char** argv) {
     sp.energy_interval = 1;
     sp.compact_interval = 0;
     sp.kinetic_interval = 0;
-    std::vector<double> hist;
-    for (int first = 1; first <= num_steps; first += (int)chunk) {
-      const int n = (int)std::min<long>(chunk, num_steps - first + 1);
-      hist.assign(3 * (size_t)n, 0.0);
-      CPIC_CALL(cpic_step(ctx, &sp, n, hist.data()));
-      for (int k = 0; k < n; ++k) {
-        const int step = first + k;
-        dump_energy_line(step, step * dt, (real_t)hist[3 * k], (real_t)hist[3 * k + 1]);
+    if (dumps) {
+      // step k is queued, then its snapshot into slot k % 2; the writer
+      // formats step k from that slot while steps k + 1, k + 2 run.  A slot
+      // is begun again only after the writer is done with it.
+      std::mutex m;
+      std::condition_variable cv;
+      int begun = 0, written = 0;
+      bool failed = false;
+      std::thread writer([&] {
+        for (int step = 1; step <= num_steps; ++step) {
+          {
+            std::unique_lock<std::mutex> lk(m);
+            cv.wait(lk, [&] { return begun >= step || failed; });
+            if (failed) return;
+          }
+          cpic_snapshot sn{};
+          const int rc = cpic_snapshot_wait(ctx, step % 2, &sn);
+          if (rc == CPIC_OK) write_dump(sn, step, true);
+          std::lock_guard<std::mutex> lk(m);
+          if (rc != CPIC_OK) {
+            std::fprintf(stderr, "cbnpic: cpic_snapshot_wait failed (%d): %s\n", rc,
+                         cpic_last_error());
+            failed = true;
+          } else {
+            written = step;
+          }
+          cv.notify_all();
+          if (failed) return;
+        }
+      });
+      int rc = CPIC_OK;
+      const char* what = "";
+      for (int step = 1; step <= num_steps && rc == CPIC_OK; ++step) {
+        if ((rc = cpic_step_async(ctx, &sp, 1)) != CPIC_OK) {
+          what = "cpic_step_async";
+          break;
+        }
+        std::unique_lock<std::mutex> lk(m);
+        cv.wait(lk, [&] { return written >= step - 2 || failed; });
+        if (failed) break;
+        if ((rc = cpic_snapshot_begin(ctx, step % 2)) != CPIC_OK) {
+          what = "cpic_snapshot_begin";
+          break;
+        }
+        begun = step;
+        cv.notify_all();
       }
-      if (n == 1) dump_step(ctx, first);
+      {
+        std::lock_guard<std::mutex> lk(m);
+        if (rc != CPIC_OK) failed = true;
+        cv.notify_all();
+      }
+      writer.join();
+      if (rc != CPIC_OK) die(what, rc);
+      if (failed) std::exit(1);
+    } else {
+      std::vector<double> hist;
+      for (int first = 1; first <= num_steps; first += (int)chunk) {
+        const int n = (int)std::min<long>(chunk, num_steps - first + 1);
+        hist.assign(3 * (size_t)n, 0.0);
+        CPIC_CALL(cpic_step(ctx, &sp, n, hist.data()));
+        for (int k = 0; k < n; ++k) {
+          const int step = first + k;
+          dump_energy_line(step, step * dt, (real_t)hist[3 * k], (real_t)hist[3 * k + 1]);
+        }
+      }
     }
+    CPIC_CALL(cpic_synchronize(ctx));
   }
+  std::fprintf(stderr, "#step loop: %.6f s for %d steps\n",
+               std::chrono::duration<double>(std::chrono::steady_clock::now() - loop_start)
+                   .count(),
+               num_steps);
   if (fptr) std::fclose(fptr);
   if (fpfd) std::fclose(fpfd);
   CPIC_CALL(cpic_gauss_residual(ctx, dx, dy, dz, eps0, qsp, CPIC_GAUSS_CHANGE, gauss, nullptr));

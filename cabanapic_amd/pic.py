@@ -137,6 +137,10 @@ class Simulation:
         check(self.lib.cpic_column_rows(self._h, ctypes.byref(r)))
         return r.value
 
+    def reserve_column_rows(self, rows):
+        """Grows the particle columns to at least `rows` rows on the device."""
+        check(self.lib.cpic_reserve_column_rows(self._h, int(rows)))
+
     def upload_fields(self, fields):
         f = self._arr(fields)
         if f.size != 9 * self.ncells:

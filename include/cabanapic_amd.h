@@ -116,6 +116,13 @@ int cpic_download_particle_tags(cpic_context *ctx, int64_t n, int64_t *tags);
 int cpic_num_particles(cpic_context *ctx, int64_t *n);
 /* Rows of each cell's particle column (no reference counterpart). */
 int cpic_column_rows(cpic_context *ctx, int64_t *rows);
+/* Grows the particle columns to at least `rows` rows (rounded up to 64) on
+ * the device, as the automatic regrowth before a push does when a column
+ * nears its capacity: a new column set is allocated and the occupied rows of
+ * every column are packed into it in row order.  No-op when the columns
+ * already hold that many rows.  No reference counterpart (Cabana's AoSoA is
+ * flat); for callers that expect particles to bunch.  Synchronous. */
+int cpic_reserve_column_rows(cpic_context *ctx, int64_t rows);
 int cpic_upload_fields(cpic_context *ctx, const void *fields);
 int cpic_download_fields(cpic_context *ctx, void *fields);
 int cpic_upload_interpolators(cpic_context *ctx, const void *interp);
@@ -236,6 +243,43 @@ int cpic_sort_particles(cpic_context *ctx);
  * reference synchronises at every dump).  Synchronous at return. */
 int cpic_step(cpic_context *ctx, const cpic_step_params *sp, int64_t nsteps,
               double *hist);
+
+/* cpic_step without the history and without the synchronisation at return:
+ * the steps are queued on the context's stream.  Energies the steps take
+ * stay on the device, the last ones in the context (cpic_snapshot_begin
+ * copies them).  A device error of these steps fails the push two steps
+ * later or the next synchronous call.  Stage timing of these steps is
+ * collected by the next synchronous call. */
+int cpic_step_async(cpic_context *ctx, const cpic_step_params *sp,
+                    int64_t nsteps);
+
+/* Per-step dumps without a stall (the partloc / ex1d dumps of
+ * example/example.cpp:59-60,130-132,273-277, whose data is that of
+ * src/helpers.h:31-70 and src/fields.h:318-350, and the energy line of
+ * src/fields.h:741-762).  cpic_snapshot_begin queues, on the context's
+ * stream after the work already queued, a gather of: the in-cell offset x,
+ * the momentum ux and the voxel of every particle in upload order (the order
+ * of the particle arrays the reference dumps); ex at voxels (i, 1, 1),
+ * i = 1..nx; and the last E, B and kinetic energies; then a copy of them into
+ * pinned host memory the library owns, on a separate copy stream, and
+ * returns.  cpic_snapshot_wait blocks until the copy of that slot has landed
+ * and fills *out with pointers into it, valid until the next begin on the
+ * slot.  Two slots (0, 1) let a caller format one step's dump while the next
+ * steps run; a slot must be waited for before it is begun again.  Begin and
+ * wait of one slot must not run concurrently; wait may run on another thread
+ * than the steps. */
+typedef struct cpic_snapshot {
+  int64_t n;            /* particles */
+  int64_t nx;           /* ex samples */
+  int32_t real_size;
+  const void *x;        /* in-cell offset x of particle i at [i]          */
+  const void *ux;       /* momentum ux                                    */
+  const int32_t *cell;  /* voxel                                          */
+  const void *ex_line;  /* ex at voxels (1..nx, 1, 1)                     */
+  double e_energy, b_energy, kinetic_energy; /* last ones taken (0 if none) */
+} cpic_snapshot;
+int cpic_snapshot_begin(cpic_context *ctx, int32_t slot);
+int cpic_snapshot_wait(cpic_context *ctx, int32_t slot, cpic_snapshot *out);
 
 /* ---- diagnostics --------------------------------------------------------- */
 

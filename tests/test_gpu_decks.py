@@ -191,25 +191,48 @@ def _expected_particle_line(p, dx, nx, ny, real):
     return "".join(f"{float(a):e}  {float(b):e} " for a, b in zip(x, p["ux"]))
 
 
+def _partloc(path):
+    """{step: values} of a partloc file (time first, then x, ux pairs)."""
+    lines = path.read_text().split("\n")
+    out = {}
+    for i in range(0, len(lines) - 1, 2):
+        if not lines[i].startswith("#step="):
+            break
+        out[int(lines[i][6:])] = np.array(lines[i + 1].split(), float)
+    return lines, out
+
+
 @pytest.mark.parametrize("real", ["float", "double"])
 def test_reference_dumps(real, tmp_path):
     """--dumps writes the reference's partloc and ex1d files
     (example/example.cpp:59-60,130-132,273-277): a step-0 particle line
     equal to the reference's formula applied to the deck's initial state,
-    then one block per step with every particle and the nx ex samples."""
-    r = run("beam_1d_custom_init", real, tmp_path, "--dumps")
+    then one block per step with every particle and the nx ex samples.
+    The dumps run queues the steps without waiting and formats step k from a
+    snapshot while later steps run: its energies.txt equals the plain fused
+    run's, its partloc and ex1d equal those of the stage-by-stage run (which
+    takes each snapshot synchronously), and the particles keep the order of
+    the initial state (every particle moves by at most c dt per step)."""
+    deck = "beam_1d_custom_init"
+    r = run(deck, real, tmp_path, "--dumps")
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    k, p = host_driver.dump("beam_1d_custom_init", real, str(tmp_path / "init.bin"))
+    k, p = host_driver.dump(deck, real, str(tmp_path / "init.bin"))
     n = p["cell"].size
-    lines = (tmp_path / "partloc").read_text().split("\n")
+    lines, dumps = _partloc(tmp_path / "partloc")
     assert lines[0] == "#step=0"
     assert lines[1] == "0 " + _expected_particle_line(p, k["dx"], 32, 1, real)
+    assert sorted(dumps) == list(range(31))
     for step in range(1, 31):
-        head, body = lines[2 * step], lines[2 * step + 1]
-        assert head == f"#step={step}"
-        vals = body.split()
+        vals = dumps[step]
         assert len(vals) == 1 + 2 * n and float(vals[0]) == pytest.approx(step * k["dt"], rel=1e-6)
-        assert np.all(np.isfinite(np.array(vals, float)))
+        assert np.all(np.isfinite(vals))
+    # particle order: x of particle i moves by at most c dt (c = 1) per step,
+    # across the periodic wrap of the 32-cell box
+    length = 32 * k["dx"]
+    for step in range(1, 31):
+        d = np.abs(dumps[step][1::2] - dumps[step - 1][1::2])
+        d = np.minimum(d, length - d)
+        assert d.max() <= k["dt"] * (1 + 1e-5), (step, d.max())
     blocks = (tmp_path / "ex1d").read_text().split("#step=")[1:]
     assert len(blocks) == 30
     for step, b in enumerate(blocks, 1):
@@ -217,6 +240,23 @@ def test_reference_dumps(real, tmp_path):
         assert b.startswith(f"{step}\n") and len(rows) == 32
         xs = np.array([float(r_[0]) for r_ in rows])
         assert np.allclose(xs, (np.arange(1, 33) - 0.5) * k["dx"], rtol=1e-6)
+
+    plain = tmp_path / "plain"
+    plain.mkdir()
+    r = run(deck, real, plain)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    assert (plain / "energies.txt").read_text() == (tmp_path / "energies.txt").read_text()
+    staged = tmp_path / "staged"
+    staged.mkdir()
+    r = run(deck, real, staged, "--stages", "--dumps")
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    _, sd = _partloc(staged / "partloc")
+    assert sorted(sd) == sorted(dumps)
+    for step in dumps:
+        assert np.array_equal(sd[step], dumps[step]), step
+    ex_a = np.array((tmp_path / "ex1d").read_text().replace("#step=", " ").split(), float)
+    ex_b = np.array((staged / "ex1d").read_text().replace("#step=", " ").split(), float)
+    assert np.array_equal(ex_a, ex_b)
 
 
 def test_column_deck_host_driver_matches_python_deck(tmp_path):
