@@ -49,6 +49,14 @@ class StepParams(ctypes.Structure):
         ("compact_interval", ctypes.c_int32), ("kinetic_interval", ctypes.c_int32)]
 
 
+class Snapshot(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nx", ctypes.c_int64), ("real_size", ctypes.c_int32),
+                ("x", ctypes.c_void_p), ("ux", ctypes.c_void_p),
+                ("cell", ctypes.POINTER(ctypes.c_int32)), ("ex_line", ctypes.c_void_p),
+                ("e_energy", ctypes.c_double), ("b_energy", ctypes.c_double),
+                ("kinetic_energy", ctypes.c_double)]
+
+
 class PlasmaSpec(ctypes.Structure):
     _fields_ = [("seed", ctypes.c_uint64), ("nx", ctypes.c_int64), ("ny", ctypes.c_int64),
                 ("nz", ctypes.c_int64), ("nppc", ctypes.c_int64), ("weight", ctypes.c_double),
@@ -100,6 +108,9 @@ SIGNATURES = {
     "cpic_kinetic_energy": (ctypes.c_int, [_P, _D, _D, ctypes.POINTER(_D)]),
     "cpic_sort_particles": (ctypes.c_int, [_P]),
     "cpic_step": (ctypes.c_int, [_P, ctypes.POINTER(StepParams), _I64, _P]),
+    "cpic_step_async": (ctypes.c_int, [_P, ctypes.POINTER(StepParams), _I64]),
+    "cpic_snapshot_begin": (ctypes.c_int, [_P, _I32]),
+    "cpic_snapshot_wait": (ctypes.c_int, [_P, _I32, ctypes.POINTER(Snapshot)]),
     "cpic_set_timing": (ctypes.c_int, [_P, _I32]),
     "cpic_set_timing_stages": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "cpic_stage_times": (ctypes.c_int, [_P, _P, _P]),

@@ -254,6 +254,36 @@ class Simulation:
                                  _ptr(hist) if history else None))
         return hist
 
+    def step_async(self, params, nsteps):
+        """Queues nsteps of the loop body without waiting for them (no
+        history; device errors surface at the next synchronous call)."""
+        if isinstance(params, dict):
+            params = make_step_params(**params)
+        check(self.lib.cpic_step_async(self._h, ctypes.byref(params), int(nsteps)))
+
+    def snapshot_begin(self, slot):
+        """Queues a dump snapshot into slot 0 or 1 (see snapshot_wait)."""
+        check(self.lib.cpic_snapshot_begin(self._h, int(slot)))
+
+    def snapshot_wait(self, slot):
+        """Waits for the snapshot of the slot; returns copies of its data:
+        x (in-cell offset), ux and cell of every particle in upload order,
+        ex_line (ex at voxels (1..nx, 1, 1)) and the last energies."""
+        sn = _capi.Snapshot()
+        check(self.lib.cpic_snapshot_wait(self._h, int(slot), ctypes.byref(sn)))
+        n, nx = int(sn.n), int(sn.nx)
+        rt = ctypes.c_float if sn.real_size == 4 else ctypes.c_double
+
+        def arr(ptr, count, ct):
+            if count == 0:
+                return np.zeros(0, np.dtype(ct))
+            return np.ctypeslib.as_array(ctypes.cast(ptr, ctypes.POINTER(ct)), (count,)).copy()
+
+        return {"x": arr(sn.x, n, rt), "ux": arr(sn.ux, n, rt),
+                "cell": arr(sn.cell, n, ctypes.c_int32), "ex_line": arr(sn.ex_line, nx, rt),
+                "e_energy": sn.e_energy, "b_energy": sn.b_energy,
+                "kinetic_energy": sn.kinetic_energy}
+
     # -- diagnostics ------------------------------------------------------------
     def set_timing(self, enabled, stages=None):
         """Times every stage (enabled), none, or the named stages only."""
