@@ -56,8 +56,10 @@ def run_gpu(deck, steps, warmup, compact_interval, history):
     from cabanapic_amd import Simulation
     parts = deck.load_particles()
     sim = Simulation(deck.nx, deck.ny, deck.nz, deck.num_particles, deck.R)
+    tu = time.perf_counter()
     sim.upload_particles(*[parts[k] for k in ("x", "y", "z", "ux", "uy", "uz", "w")],
                          parts["cell"])
+    t_upload = time.perf_counter() - tu
     del parts
     fields = deck.load_fields()
     if fields is not None:
@@ -86,6 +88,9 @@ def run_gpu(deck, steps, warmup, compact_interval, history):
     change, rho = sim.gauss_residual(*gargs, baseline=False)
     stages["gauss_residual_change_rel"] = change / rho
     errs = sim.device_errors()
+    td = time.perf_counter()
+    sim.download_particles()
+    stages["host_transfer_s"] = (t_upload, time.perf_counter() - td)
     sim.close()
     return t1 - t0, hist, stages, errs
 
@@ -349,10 +354,15 @@ def main():
     step_gbs = (bp + bc) / (t32 / args.steps) / 1e9
 
     gauss32 = st32.pop("gauss_residual_change_rel")
+    t_up, t_down = st32.pop("host_transfer_s")
     extra = {"stage_ms_per_step": {k: round(v[1] / max(v[0], 1), 4) for k, v in st32.items()
                                    if v[0]},
              "gauss_residual_change_rel": gauss32,
              "step_hbm_gbs": round(step_gbs, 1),
+             # particles handed over in host buffers: upload (binning into
+             # columns included) before and download after the timed steps
+             "upload_s": round(t_up, 4), "download_s": round(t_down, 4),
+             "pcie_inclusive_value": np_ * args.steps / (t_up + t32 + t_down),
              "step_roofline_frac": round(step_gbs / HBM_PEAK_GBS, 4)}
 
     if args.fp64_steps > 0:
