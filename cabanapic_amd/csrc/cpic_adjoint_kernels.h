@@ -563,9 +563,9 @@ __global__ void __launch_bounds__(kBlock) k_push_adjoint(AdjointPush<R> a) {
         int K = 0;
         {
           R pos[3] = {dx, dy, dz}, rem[3] = {dd[0], dd[1], dd[2]};
-          R scratch[12];
+          R dsp[3], mid[3], v5;
           for (; K < kMaxStreaks; ++K) {
-            if (streak(pos, rem, q, scratch) < 0) break;
+            if (streak_geometry(pos, rem, q, dsp, mid, v5) < 0) break;
           }
           ++K;
           if (K > kMaxStreaks) {
@@ -578,8 +578,9 @@ __global__ void __launch_bounds__(kBlock) k_push_adjoint(AdjointPush<R> a) {
           // inputs and cell of streak s
           R pos[3] = {dx, dy, dz}, rem[3] = {dd[0], dd[1], dd[2]};
           int cs = ci;
-          R scratch[12];
-          for (int k = 0; k < s; ++k) cs = face_neighbor(a.g, cs, streak(pos, rem, q, scratch));
+          R dsp[3], mid[3], v5;
+          for (int k = 0; k < s; ++k)
+            cs = face_neighbor(a.g, cs, streak_geometry(pos, rem, q, dsp, mid, v5));
           load_lacc(a, interior_to_voxel(a.g, cs), lacc);
           streak_bwd(pos, rem, q, lacc, lpos, lrem);
         }

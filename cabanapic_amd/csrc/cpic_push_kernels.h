@@ -269,16 +269,18 @@ __device__ __forceinline__ void add12(LaneAcc<R, K>& acc, const R o[12]) {
   for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
 }
 
-// One iteration of the move_p loop body (move_p.h:101-209, 218-227,
-// 368-370) for a particle in the calling lane's cell: the streak up to the
-// first face or the end of the track.  Adds the streak's 12 contributions to
-// acc, advances pos/rem, and returns the face crossed (0..5: -x,-y,-z,
-// +x,+y,+z) or -1 when the track ends in the cell.  On a crossing pos is
-// already expressed in the neighbour's coordinates (the crossed coordinate
-// flipped to the entry face).
-template <class R, class A>
-__device__ __forceinline__ int streak(R pos[3], R rem[3], R q, A& acc) {
-  R dir[3], len3[3], dsp[3], mid[3];
+// The geometry of one iteration of the move_p loop body (move_p.h:101-209,
+// 218-227, 368-370) for a particle in the calling lane's cell: the streak
+// up to the first face or the end of the track.  Returns the streak's
+// displacement dsp and midpoint mid and its correction term v5 (the inputs
+// of its 12 contributions, streak_values), advances pos/rem, and returns the
+// face crossed (0..5: -x,-y,-z, +x,+y,+z) or -1 when the track ends in the
+// cell.  On a crossing pos is already expressed in the neighbour's
+// coordinates (the crossed coordinate flipped to the entry face).
+template <class R>
+__device__ __forceinline__ int streak_geometry(R pos[3], R rem[3], R q, R dsp[3], R mid[3],
+                                               R& v5) {
+  R dir[3], len3[3];
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     dsp[k] = rem[k];
@@ -297,10 +299,7 @@ __device__ __forceinline__ int streak(R pos[3], R rem[3], R q, A& acc) {
     dsp[k] *= len;
     mid[k] += dsp[k];
   }
-  const R v5 = streak_v5(q, dsp[0], dsp[1], dsp[2]);
-  R o[12];
-  streak_values(q, dsp[0], dsp[1], dsp[2], mid[0], mid[1], mid[2], v5, o);
-  add12(acc, o);
+  v5 = streak_v5(q, dsp[0], dsp[1], dsp[2]);
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     rem[k] -= dsp[k];
@@ -312,6 +311,18 @@ __device__ __forceinline__ int streak(R pos[3], R rem[3], R q, A& acc) {
   else if (axis == 1) pos[1] = -side;
   else pos[2] = -side;
   return axis + ((side > 0) ? 3 : 0);
+}
+
+// One whole iteration of the move_p loop body: streak_geometry, then the
+// streak's 12 contributions added to acc.
+template <class R, class A>
+__device__ __forceinline__ int streak(R pos[3], R rem[3], R q, A& acc) {
+  R dsp[3], mid[3], v5;
+  const int d = streak_geometry(pos, rem, q, dsp, mid, v5);
+  R o[12];
+  streak_values(q, dsp[0], dsp[1], dsp[2], mid[0], mid[1], mid[2], v5, o);
+  add12(acc, o);
+  return d;
 }
 
 template <class R> struct ColState {
@@ -547,12 +558,11 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
       const R q = w * a.qsp;
       R px, py, pz;
       int d = -1;
+      // the deposit's inputs: the whole move (it stays in the cell,
+      // push.h:218-232) or the first streak of move_p
+      R s0 = ddx, s1 = ddy, s2 = ddz, m0 = mx, m1 = my, m2 = mz, v5;
       if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
-        const R v5 = q * ddx * ddy * ddz * one_third;
-        R o[12];
-        streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
+        v5 = q * ddx * ddy * ddz * one_third;
         px = nx;
         py = ny;
         pz = nz;
@@ -560,11 +570,24 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
         ++nmov;
         R pos[3] = {dx, dy, dz};
         R rem[3] = {ddx, ddy, ddz};
-        d = streak(pos, rem, q, acc);
+        R dsp[3], mid[3];
+        d = streak_geometry(pos, rem, q, dsp, mid, v5);
+        s0 = dsp[0];
+        s1 = dsp[1];
+        s2 = dsp[2];
+        m0 = mid[0];
+        m1 = mid[1];
+        m2 = mid[2];
         // a particle that leaves is stored with its position before the push
         px = d < 0 ? pos[0] : dx;
         py = d < 0 ? pos[1] : dy;
         pz = d < 0 ? pos[2] : dz;
+      }
+      {
+        R o[12];
+        streak_values(q, s0, s1, s2, m0, m1, m2, v5, o);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
       }
       const int row = ns++;
       if ((row >> 6) != cw) {
@@ -856,12 +879,11 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
       const R nx = mx + ddx, ny = my + ddy, nz = mz + ddz;
       const R q = w * a.qsp;
       R px, py, pz;
+      // the deposit's inputs: the whole move (it stays in the cell,
+      // push.h:218-232) or the first streak of move_p
+      R s0 = ddx, s1 = ddy, s2 = ddz, m0 = mx, m1 = my, m2 = mz, v5;
       if (nx <= one && ny <= one && nz <= one && -nx <= one && -ny <= one && -nz <= one) {
-        const R v5 = q * ddx * ddy * ddz * one_third;
-        R o[12];
-        streak_values(q, ddx, ddy, ddz, mx, my, mz, v5, o);
-#pragma unroll
-        for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
+        v5 = q * ddx * ddy * ddz * one_third;
         px = nx;
         py = ny;
         pz = nz;
@@ -869,7 +891,14 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
         ++nmov;
         R pos[3] = {dx, dy, dz};
         R rem[3] = {ddx, ddy, ddz};
-        const int d = streak(pos, rem, q, acc);
+        R dsp[3], mid[3];
+        const int d = streak_geometry(pos, rem, q, dsp, mid, v5);
+        s0 = dsp[0];
+        s1 = dsp[1];
+        s2 = dsp[2];
+        m0 = mid[0];
+        m1 = mid[1];
+        m2 = mid[2];
         // a particle that leaves is stored with its position before the push
         px = d < 0 ? pos[0] : dx;
         py = d < 0 ? pos[1] : dy;
@@ -881,6 +910,12 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
         dw3 |= d == 3 ? bit : 0ull;
         dw4 |= d == 4 ? bit : 0ull;
         dw5 |= d == 5 ? bit : 0ull;
+      }
+      {
+        R o[12];
+        streak_values(q, s0, s1, s2, m0, m1, m2, v5, o);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) acc.add(k, o[k]);
       }
       m_out |= 1ull << b;
       hi = j + 1;
@@ -1055,11 +1090,12 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
       R rem[3];
       displacement(ux, uy, uz, a.cdt_dx, a.cdt_dy, a.cdt_dz, rem[0], rem[1], rem[2]);
       const R q = w8 * a.qsp;
-      R scratch[12];
-#pragma unroll
-      for (int k = 0; k < 12; ++k) scratch[k] = 0;
-      // the first streak, deposited by the neighbour in pass 1
-      if (streak(pos, rem, q, scratch) < 0) atomicOr(a.err, kErrBadCell);
+      // the first streak, deposited by the neighbour in pass 1: its
+      // geometry only
+      {
+        R dsp[3], mid[3], v5;
+        if (streak_geometry(pos, rem, q, dsp, mid, v5) < 0) atomicOr(a.err, kErrBadCell);
+      }
       const int d2n = streak(pos, rem, q, acc);
       if (d2n < 0) {
         const int row = rows.next(a.cs, ci);

@@ -74,9 +74,16 @@ template <class R> struct DualV4 {
 };
 
 template <class R> struct TangentPush {
-  R* part;           // [7][stride]: x, y, z, ux, uy, uz, w
-  int* cell;         // voxel of each particle
-  double* tpart;     // [6][stride]: tangents of x, y, z, ux, uy, uz
+  // the particles before the push (sorted by cell, so a cell's particles
+  // sit in consecutive lanes) and after it, at the same index
+  const R* src_part;      // [7][stride]: x, y, z, ux, uy, uz, w
+  const int* src_cell;    // voxel of each particle
+  const double* src_tpart;  // [6][stride]: tangents of x, y, z, ux, uy, uz
+  const int* src_tag;     // upload index of each particle
+  R* part;
+  int* cell;
+  double* tpart;
+  int* tag;
   long stride;
   int n;
   const R* interp;          // primal interpolators
@@ -140,12 +147,17 @@ __device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a, in
   typedef typename Vec4T<R>::type V4;
   typedef double T4 __attribute__((ext_vector_type(4)));
   const long S = a.stride;
-  const int v0 = a.cell[p];
+  const int v0 = a.src_cell[p];
+  a.tag[p] = a.src_tag[p];
   const int X = a.g.sy, Y = a.g.sz / a.g.sy;
   const int ix = v0 % X - 1, iy = (v0 / X) % Y - 1, iz = v0 / a.g.sz - 1;
   if (v0 < 0 || v0 >= a.ncells || ix < 0 || ix >= a.g.nx || iy < 0 || iy >= a.g.ny || iz < 0 ||
       iz >= a.g.nz) {
     atomicOr(a.err, kErrBadCell);
+    // carried over unchanged
+    for (int k = 0; k < 7; ++k) a.part[k * S + p] = a.src_part[k * S + p];
+    for (int k = 0; k < 6; ++k) a.tpart[k * S + p] = a.src_tpart[k * S + p];
+    a.cell[p] = v0;
     return -1;
   }
   int ci = ix + a.g.nx * (iy + a.g.ny * iz);
@@ -162,13 +174,13 @@ __device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a, in
     e[k].z = D(fv.z, tv.z);
     e[k].w = D(fv.w, tv.w);
   }
-  const D dx(a.part[p], a.tpart[p]);
-  const D dy(a.part[S + p], a.tpart[S + p]);
-  const D dz(a.part[2 * S + p], a.tpart[2 * S + p]);
-  D ux(a.part[3 * S + p], a.tpart[3 * S + p]);
-  D uy(a.part[4 * S + p], a.tpart[4 * S + p]);
-  D uz(a.part[5 * S + p], a.tpart[5 * S + p]);
-  const R w = a.part[6 * S + p];
+  const D dx(a.src_part[p], a.src_tpart[p]);
+  const D dy(a.src_part[S + p], a.src_tpart[S + p]);
+  const D dz(a.src_part[2 * S + p], a.src_tpart[2 * S + p]);
+  D ux(a.src_part[3 * S + p], a.src_tpart[3 * S + p]);
+  D uy(a.src_part[4 * S + p], a.src_tpart[4 * S + p]);
+  D uz(a.src_part[5 * S + p], a.src_tpart[5 * S + p]);
+  const R w = a.src_part[6 * S + p];
 
   boris(e[0], e[1], e[2], e[3], e[4], D(a.qdt_2mc, 0.), dx, dy, dz, ux, uy, uz);
   D ddx, ddy, ddz;
@@ -216,8 +228,44 @@ __device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a, in
   a.tpart[3 * S + p] = ux.t;
   a.tpart[4 * S + p] = uy.t;
   a.tpart[5 * S + p] = uz.t;
+  a.part[6 * S + p] = w;
   a.cell[p] = interior_to_voxel(a.g, ci);
   return v0;
+}
+
+// Counting sort of the particles by voxel before each tangent push, so that
+// the run sums of the push's own-cell deposits see a cell's particles in
+// consecutive lanes (particles keep drifting out of the order of their
+// upload).  Count: particles per voxel (an invalid voxel counts in bin 0
+// and is flagged by the push).
+__global__ void __launch_bounds__(kBlock)
+k_tangent_bin_count(const int* __restrict__ cell, int n, int ncells, int* __restrict__ bins) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int v = cell[p];
+  atomicAdd(bins + ((v >= 0 && v < ncells) ? v : 0), 1);
+}
+
+// Scatter: every particle to the next free position of its voxel's range
+// (start: the exclusive scan of the counts, advanced by this kernel); the
+// order within a voxel is unspecified.
+template <class R>
+__global__ void __launch_bounds__(kBlock)
+k_tangent_bin_scatter(const R* __restrict__ part, const int* __restrict__ cell,
+                      const double* __restrict__ tpart, const int* __restrict__ tag, int n,
+                      long stride, int ncells, int* __restrict__ start, R* __restrict__ part_out,
+                      int* __restrict__ cell_out, double* __restrict__ tpart_out,
+                      int* __restrict__ tag_out) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const int v = cell[p];
+  const int o = atomicAdd(start + ((v >= 0 && v < ncells) ? v : 0), 1);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) part_out[k * stride + o] = part[k * stride + p];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) tpart_out[k * stride + o] = tpart[k * stride + p];
+  cell_out[o] = v;
+  tag_out[o] = tag[p];
 }
 
 // One thread per particle: the push of src/push.h:65-295 with the move_p
