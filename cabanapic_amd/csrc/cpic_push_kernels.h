@@ -115,6 +115,24 @@ __device__ __forceinline__ int face_neighbor(const CellGrid& g, int ci, int d) {
   return ix + g.nx * (iy + g.ny * iz);
 }
 
+// All six face neighbours of interior cell ci, nb[d] = face_neighbor(g, ci,
+// d), and its voxel, from one set of divisions.
+__device__ __forceinline__ void face_neighbors6(const CellGrid& g, int ci, int nb[6],
+                                                int& voxel) {
+  const int ix = ci % g.nx;
+  const int rest = ci / g.nx;
+  const int iy = rest % g.ny;
+  const int iz = rest / g.ny;
+  const int sy = g.nx, sz = g.nx * g.ny;
+  nb[0] = ix > 0 ? ci - 1 : ci + (g.nx - 1);
+  nb[3] = ix < g.nx - 1 ? ci + 1 : ci - (g.nx - 1);
+  nb[1] = iy > 0 ? ci - sy : ci + (g.ny - 1) * sy;
+  nb[4] = iy < g.ny - 1 ? ci + sy : ci - (g.ny - 1) * sy;
+  nb[2] = iz > 0 ? ci - sz : ci + (g.nz - 1) * sz;
+  nb[5] = iz < g.nz - 1 ? ci + sz : ci - (g.nz - 1) * sz;
+  voxel = (ix + 1) + g.sy * (iy + 1) + g.sz * (iz + 1);
+}
+
 // Wave-wide reductions; the result is read back as a scalar, so the
 // compiler knows it is the same in every lane (branches and loops on it are
 // then scalar, not exec-masked).
@@ -696,14 +714,17 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   const size_t NC = a.g.nc;
   int s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
   int c0 = 0, c1 = 0, c2 = 0, c3 = 0, c4 = 0, c5 = 0;
+  int v = 0;
   if (valid) {
     // the source of a record of direction d sits across the opposite face
-    s0 = face_neighbor(a.g, ci, 3);
-    s1 = face_neighbor(a.g, ci, 4);
-    s2 = face_neighbor(a.g, ci, 5);
-    s3 = face_neighbor(a.g, ci, 0);
-    s4 = face_neighbor(a.g, ci, 1);
-    s5 = face_neighbor(a.g, ci, 2);
+    int nb[6];
+    face_neighbors6(a.g, ci, nb, v);
+    s0 = nb[3];
+    s1 = nb[4];
+    s2 = nb[5];
+    s3 = nb[0];
+    s4 = nb[1];
+    s5 = nb[2];
     c0 = a.in.cnt[0 * NC + s0];
     c1 = a.in.cnt[1 * NC + s1];
     c2 = a.in.cnt[2 * NC + s2];
@@ -715,7 +736,6 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   const int tmax = wave_max_int(total);
   int kd[6] = {0, 0, 0, 0, 0, 0};
   if (tmax > 0) {
-    const int v = valid ? interior_to_voxel(a.g, ci) : 0;
     RowCursor<R> rows;
     {
       const int l = valid ? min(a.cs.cnt_out[ci], a.cs.D) : 0;
@@ -1015,15 +1035,17 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
   // a particle that crossed face d came from the neighbour across the
   // opposite face
   int s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
+  int v = 0;
   if (valid) {
-    s0 = face_neighbor(a.g, ci, 3);
-    s1 = face_neighbor(a.g, ci, 4);
-    s2 = face_neighbor(a.g, ci, 5);
-    s3 = face_neighbor(a.g, ci, 0);
-    s4 = face_neighbor(a.g, ci, 1);
-    s5 = face_neighbor(a.g, ci, 2);
+    int nb[6];
+    face_neighbors6(a.g, ci, nb, v);
+    s0 = nb[3];
+    s1 = nb[4];
+    s2 = nb[5];
+    s3 = nb[0];
+    s4 = nb[1];
+    s5 = nb[2];
   }
-  const int v = valid ? interior_to_voxel(a.g, ci) : 0;
   RowCursor<R> rows;
   {
     const int l = valid ? min(a.cs.cnt_out[ci], a.cs.D) : 0;
