@@ -52,6 +52,25 @@ def measured_push_traffic(n, nppc):
     return None
 
 
+COUNTERS_PROFILE = os.path.join(ROOT, "profiles", "push_stage_weibel128x32_counters.md")
+
+
+def measured_valu_busy(n, nppc):
+    """{kernel: share of its cycles issuing VALU work} from the committed
+    rocprofv3 SQ counter summary of the push passes, for the workload it was
+    measured on; None for any other workload."""
+    if (n, nppc) != (128, 32) or not os.path.exists(COUNTERS_PROFILE):
+        return None
+    out = {}
+    with open(COUNTERS_PROFILE) as f:
+        for line in f:
+            if line.startswith("| `cpic::k_") and line.rstrip().endswith("% |"):
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                name = cells[0].strip("`").split("::")[1].split("<")[0]
+                out[name] = round(float(cells[-1].rstrip(" %")) / 100, 3)
+    return out or None
+
+
 def run_gpu(deck, steps, warmup, compact_interval, history):
     from cabanapic_amd import Simulation
     parts = deck.load_particles()
@@ -359,6 +378,8 @@ def main():
                                    if v[0]},
              "gauss_residual_change_rel": gauss32,
              "step_hbm_gbs": round(step_gbs, 1),
+             # the push passes are co-bound by VALU issue (SQ counters)
+             "push_valu_busy": measured_valu_busy(args.n, args.nppc),
              # particles handed over in host buffers: upload (binning into
              # columns included) before and download after the timed steps
              "upload_s": round(t_up, 4), "download_s": round(t_down, 4),
