@@ -6,6 +6,10 @@ full Weibel configuration.
   sum of the oracle's own FP32 contributions (summed by the oracle in FP64)
   to a few roundings of the total, where the plain FP32 sum is off by up to
   one rounding per addition.
+* 100 steps of the Weibel deck (128^3 cells x 32 particles per cell) run by
+  the oracle in FP64 (16 threads) from the same plasma: the GPU's FP64 field
+  energies follow it to the order of the sums (9e-14 measured, bound
+  1e-11), and its FP32 ones within 1e-5 (2e-7 measured).
 * 2500 steps of the Weibel deck (128^3 cells x 32 particles per cell, past
   the saturation near step 2100) in FP64, FP32 and compensated FP32 from the
   same initial plasma: the B-energy growth curve of both FP32 runs against
@@ -17,8 +21,9 @@ import json
 import numpy as np
 import pytest
 
+import oracle_lib
 import states
-from cabanapic_amd import precision
+from cabanapic_amd import decks, precision
 from test_gpu_parity import assert_particles_bitwise, sim_from_state
 
 pytestmark = pytest.mark.gpu
@@ -102,3 +107,33 @@ def test_weibel_fp32_against_fp64_to_saturation():
         assert f["growth_rate_rel_err"] < 1e-3, (name, f)
         assert f["saturation_rel_err"] < 1e-3, (name, f)
         assert abs(f["saturation_step"] - f["saturation_step_fp64"]) <= 10, (name, f)
+
+
+@pytest.mark.timeout(900)
+def test_weibel_against_fp64_oracle_run():
+    steps = 100
+    d64 = decks.weibel_3d(np.float64)
+    parts = d64.load_particles()
+    st = oracle_lib.OracleState(d64.nx, d64.ny, d64.nz, d64.num_particles, np.float64)
+    for m in states.MEMBERS:
+        st.p[m] = parts[m]
+    st.cell = parts["cell"]
+    del parts
+    k = d64.driver_constants()
+    consts = [k[n] for n in ("qdt_2mc", "cdt_dx", "cdt_dy", "cdt_dz", "qsp", "dx", "dy", "dz",
+                             "dt", "px", "py", "pz", "dt_eps0", "me", "c")]
+    print(f"oracle FP64 run of {steps} steps on {d64.num_particles} particles", flush=True)
+    h_o, _ = st.run(consts, steps, nthreads=16)
+    del st
+    h64, _ = precision.run(d64, steps)
+    h32, _ = precision.run(decks.weibel_3d(np.float32), steps)
+    tot_o = h_o[:, 0] + h_o[:, 1]
+    r64 = np.abs((h64[:, 0] + h64[:, 1]) - tot_o) / tot_o
+    r32 = np.abs((h32[:, 0] + h32[:, 1]) - tot_o) / tot_o
+    b64 = np.abs(np.log(h64[:, 1] / h_o[:, 1]))
+    b32 = np.abs(np.log(h32[:, 1] / h_o[:, 1]))
+    print(f"Weibel, {steps} steps against the oracle's FP64 run: field energy max rel. "
+          f"difference GPU FP64 {r64.max():.2e}, GPU FP32 {r32.max():.2e}; B energy max "
+          f"|ln ratio| FP64 {b64.max():.2e}, FP32 {b32.max():.2e}")
+    assert r64.max() < 1e-11 and b64.max() < 1e-11
+    assert r32.max() < 1e-5 and b32.max() < 1e-5
