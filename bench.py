@@ -71,6 +71,28 @@ def measured_valu_busy(n, nppc):
     return out or None
 
 
+def copy_bandwidth(reps=10):
+    """Achieved bandwidth of a plain 1 GiB device-to-device copy (torch),
+    GB/s counting bytes read plus bytes written: the practical streaming
+    ceiling of the box beside the published 8 TB/s."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * 4 * n / (ms * 1e-3) / 1e9
+
+
 def run_gpu(deck, steps, warmup, compact_interval, history):
     from cabanapic_amd import Simulation
     parts = deck.load_particles()
@@ -378,6 +400,7 @@ def main():
                                    if v[0]},
              "gauss_residual_change_rel": gauss32,
              "step_hbm_gbs": round(step_gbs, 1),
+             "copy_gbs": round(copy_bandwidth(), 1),
              # the push passes are co-bound by VALU issue (SQ counters)
              "push_valu_busy": measured_valu_busy(args.n, args.nppc),
              # particles handed over in host buffers: upload (binning into
