@@ -16,6 +16,7 @@ Prints one JSON line.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -74,7 +75,9 @@ def measured_valu_busy(n, nppc):
 def copy_bandwidth(reps=10):
     """Achieved bandwidth of a plain 1 GiB device-to-device copy (torch),
     GB/s counting bytes read plus bytes written: the practical streaming
-    ceiling of the box beside the published 8 TB/s."""
+    ceiling of the box beside the published 8 TB/s.  Runs in a process of
+    its own (`--copy-bandwidth`): torch's HIP runtime does not start in a
+    process where the library's has."""
     import torch
     n = 1 << 28
     a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -91,6 +94,16 @@ def copy_bandwidth(reps=10):
     del a, b
     torch.cuda.empty_cache()
     return 2 * 4 * n / (ms * 1e-3) / 1e9
+
+
+def measured_copy_bandwidth():
+    """copy_bandwidth() from a child process; None if it fails."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-bandwidth"],
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        print(f"# copy bandwidth run failed: {r.stderr.strip()[-300:]}", file=sys.stderr)
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])["copy_gbs"]
 
 
 def run_gpu(deck, steps, warmup, compact_interval, history):
@@ -348,6 +361,9 @@ def main():
     ap.add_argument("--column-steps", type=int, default=20,
                     help="timed FP32 steps of the 192x192x64 x 32 ppc electron-column "
                          "configuration, reported beside the metric (0 = skip)")
+    ap.add_argument("--copy-bandwidth", action="store_true",
+                    help="only time a plain 1 GiB device-to-device copy (torch) and print "
+                         "its GB/s (read plus written bytes)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device-to-device copy (torch), the known-size "
                          "kernel that scales rocprofv3 FETCH_SIZE/WRITE_SIZE to bytes")
@@ -356,6 +372,10 @@ def main():
     # starts (the oracle library loads it)
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
+
+    if args.copy_bandwidth:
+        print(json.dumps({"copy_gbs": round(copy_bandwidth(), 1)}))
+        return
 
     if args.copy_calibration:
         import torch
@@ -372,6 +392,9 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {args.gpus}): the step has no "
               "multi-GPU path; its domain decomposition is a design, not code.", file=sys.stderr)
         sys.exit(2)
+
+    # before the library starts its HIP runtime in this process
+    copy_gbs = measured_copy_bandwidth()
 
     from cabanapic_amd import decks
 
@@ -400,7 +423,7 @@ def main():
                                    if v[0]},
              "gauss_residual_change_rel": gauss32,
              "step_hbm_gbs": round(step_gbs, 1),
-             "copy_gbs": round(copy_bandwidth(), 1),
+             "copy_gbs": copy_gbs,
              # the push passes are co-bound by VALU issue (SQ counters)
              "push_valu_busy": measured_valu_busy(args.n, args.nppc),
              # particles handed over in host buffers: upload (binning into

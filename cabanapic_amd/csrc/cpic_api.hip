@@ -144,6 +144,11 @@ struct cpic_context : ColumnSet {
     hipEvent_t staged = nullptr;  // on the step stream, after the gather
     hipEvent_t ready = nullptr;   // on the copy stream, after the download
     bool pending = false;         // begun, not yet waited for
+    // what the pending snapshot holds, fixed at its begin: particle count,
+    // line length, byte offsets of x, ux, cell, ex line and energies
+    int64_t n = 0;
+    int nx = 0;
+    size_t off[5] = {};
   } snap[2];
   hipStream_t copy_stream = nullptr;
 
@@ -717,6 +722,10 @@ int snapshot_begin(cpic_context* c, int slot) {
   HIP_TRY(hipStreamWaitEvent(c->copy_stream, sn.staged, 0));
   HIP_TRY(hipMemcpyAsync(sn.host, sn.dev, L.bytes, hipMemcpyDeviceToHost, c->copy_stream));
   HIP_TRY(hipEventRecord(sn.ready, c->copy_stream));
+  sn.n = c->np;
+  sn.nx = (int)c->nx;
+  const size_t off[5] = {L.x, L.ux, L.cell, L.ex, L.energies};
+  std::copy(off, off + 5, sn.off);
   sn.pending = true;
   return CPIC_OK;
 }
@@ -729,16 +738,17 @@ int snapshot_wait(cpic_context* c, int slot, cpic_snapshot* out) {
   if (!sn.pending) return fail(CPIC_ERR_INVALID, "no snapshot begun on slot %d", slot);
   HIP_TRY(hipEventSynchronize(sn.ready));
   sn.pending = false;
-  const SnapLayout L = snap_layout(c);
+  // the layout of the begin: an upload since then may have changed the
+  // particle count
   const char* h = (const char*)sn.host;
-  out->n = c->np;
-  out->nx = c->nx;
+  out->n = sn.n;
+  out->nx = sn.nx;
   out->real_size = c->real_size;
-  out->x = h + L.x;
-  out->ux = h + L.ux;
-  out->cell = (const int32_t*)(h + L.cell);
-  out->ex_line = h + L.ex;
-  const double* e = (const double*)(h + L.energies);
+  out->x = h + sn.off[0];
+  out->ux = h + sn.off[1];
+  out->cell = (const int32_t*)(h + sn.off[2]);
+  out->ex_line = h + sn.off[3];
+  const double* e = (const double*)(h + sn.off[4]);
   out->e_energy = e[0];
   out->b_energy = e[1];
   out->kinetic_energy = e[2];

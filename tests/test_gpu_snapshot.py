@@ -7,7 +7,8 @@ its ex line equals ex of the downloaded fields at voxels (1..nx, 1, 1) and
 its energies equal cpic_energies.  Steps queued with cpic_step_async equal
 cpic_step's, with the two slots used in turn while steps are queued behind
 them; a slot begun twice without a wait, or waited for without a begin, is
-an error.
+an error.  A snapshot keeps the particle count of its begin when particles
+of another count are uploaded before its wait.
 """
 import numpy as np
 import pytest
@@ -65,3 +66,23 @@ def test_snapshots_follow_queued_steps(dtype):
         with pytest.raises(CpicError):
             a.snapshot_begin(1)
         a.snapshot_wait(1)
+
+
+def test_snapshot_keeps_its_layout_across_an_upload():
+    # a snapshot begun before an upload of another particle count holds the
+    # particles of its begin; the other slot then follows the new count
+    st, _ = states.random_state(6, 5, 4, 20, np.float32, seed=93)
+    st2, _ = states.random_state(6, 5, 4, 7, np.float32, seed=94)
+    assert st2.np < st.np
+    with sim_from_state(st) as sim:
+        sim.snapshot_begin(0)
+        sim.upload_particles(*[st2.p[m] for m in ("x", "y", "z", "ux", "uy", "uz", "w")],
+                             st2.cell)
+        sim.snapshot_begin(1)
+        for slot, s in ((0, st), (1, st2)):
+            snap = sim.snapshot_wait(slot)
+            assert snap["x"].size == s.np
+            assert_bitwise(snap["x"], s.p["x"], f"x, slot {slot}")
+            assert_bitwise(snap["ux"], s.p["ux"], f"ux, slot {slot}")
+            assert_bitwise(snap["cell"], s.cell, f"cell, slot {slot}")
+        assert sim.device_errors() == 0
