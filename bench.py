@@ -96,9 +96,10 @@ def copy_bandwidth(reps=10):
     return 2 * 4 * n / (ms * 1e-3) / 1e9
 
 
-def measured_copy_bandwidth():
-    """copy_bandwidth() from a child process; None if it fails."""
-    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-bandwidth"],
+def measured_copy_bandwidth(reps):
+    """copy_bandwidth(reps) from a child process; None if it fails."""
+    r = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-bandwidth",
+                        "--copy-reps", str(reps)],
                        capture_output=True, text=True, timeout=300)
     if r.returncode != 0:
         print(f"# copy bandwidth run failed: {r.stderr.strip()[-300:]}", file=sys.stderr)
@@ -361,6 +362,9 @@ def main():
     ap.add_argument("--column-steps", type=int, default=20,
                     help="timed FP32 steps of the 192x192x64 x 32 ppc electron-column "
                          "configuration, reported beside the metric (0 = skip)")
+    ap.add_argument("--copy-reps", type=int, default=10,
+                    help="timed repetitions of the 1 GiB device copy reported as copy_gbs "
+                         "(0 = skip, e.g. under a profiler)")
     ap.add_argument("--copy-bandwidth", action="store_true",
                     help="only time a plain 1 GiB device-to-device copy (torch) and print "
                          "its GB/s (read plus written bytes)")
@@ -374,7 +378,7 @@ def main():
     os.environ.setdefault("OMP_PLACES", "cores")
 
     if args.copy_bandwidth:
-        print(json.dumps({"copy_gbs": round(copy_bandwidth(), 1)}))
+        print(json.dumps({"copy_gbs": round(copy_bandwidth(args.copy_reps), 1)}))
         return
 
     if args.copy_calibration:
@@ -394,7 +398,7 @@ def main():
         sys.exit(2)
 
     # before the library starts its HIP runtime in this process
-    copy_gbs = measured_copy_bandwidth()
+    copy_gbs = measured_copy_bandwidth(args.copy_reps) if args.copy_reps > 0 else None
 
     from cabanapic_amd import decks
 

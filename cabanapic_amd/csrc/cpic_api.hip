@@ -286,10 +286,11 @@ ColState<R> col_state(cpic_context* c, int from) {
 
 // push + move_p + deposit: pass 1, three finishing passes, spill fallback.
 // accumulate = 1 adds to the accumulators as the reference's push does;
-// accumulate = 0 overwrites them (valid right after a clear).
+// accumulate = 0 overwrites them (valid right after a clear).  gather: where
+// pass 1 takes the interpolators from (the array, or the fields).
 template <class R>
 int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, int accumulate,
-               int compact) {
+               int compact, int gather = kGatherRead) {
   StageTimer t(c, CPIC_STAGE_PUSH);
   int s;
   if ((s = fill_zero<unsigned int>(c, c->movers, 1))) return s;
@@ -307,7 +308,9 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
   p.cs = cs;
   p.dep = c->dep;
   p.dep_words = c->dep_words;
-  p.interp = (const R*)c->interp;
+  p.interp = (R*)c->interp;
+  p.fields = (const R*)c->fields;
+  p.gather = gather;
   p.acc = (R*)c->acc;
   p.g = c->cg;
   p.ncells = (int)c->ncells;
@@ -772,15 +775,20 @@ int run_steps(cpic_context* c, const cpic_step_params* sp, int64_t nsteps, doubl
   const R hpx = half * px, hpy = half * py, hpz = half * pz;
   for (int64_t step = 1; step <= nsteps; ++step) {
     int t;
-    if ((t = stage_load_interpolator<R>(c))) return t;
+    // With particles resident, pass 1 gathers each cell's interpolator from
+    // the fields itself; the array is written on the last step only, so that
+    // it holds the last push's interpolators when the call returns.
+    const bool fused = c->np > 0 && c->D > 0;
+    if (!fused && (t = stage_load_interpolator<R>(c))) return t;
     // the push overwrites every interior accumulator, so the clear is only
     // needed while ghost accumulators may hold data
     if (!c->acc_ghosts_zero && (t = stage_clear<R>(c))) return t;
     if ((t = check_headroom(c))) return t;
     const int ci = sp->compact_interval > 0 ? sp->compact_interval : kCompactInterval;
     const int compact = (c->pushes++ % ci == 0) ? 1 : 0;
+    const int gather = !fused ? kGatherRead : step == nsteps ? kGatherStore : kGatherFields;
     if ((t = stage_push<R>(c, (R)sp->qdt_2mc, (R)sp->cdt_dx, (R)sp->cdt_dy, (R)sp->cdt_dz,
-                           (R)sp->qsp, 0, compact)))
+                           (R)sp->qsp, 0, compact, gather)))
       return t;
     if ((t = note_occupancy(c))) return t;
     if ((t = stage_unload<R>(c, (R)sp->dx, (R)sp->dy, (R)sp->dz, (R)sp->dt))) return t;

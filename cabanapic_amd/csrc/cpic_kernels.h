@@ -52,8 +52,60 @@ __device__ __forceinline__ int voxel(const Grid& g, int x, int y, int z) {
 }
 
 // ---------------------------------------------------------------------------
-// load_interpolator_array (src/interpolator.cpp:4-124): one thread per
-// interior cell, x fastest so neighbouring lanes read neighbouring voxels.
+// load_interpolator_array (src/interpolator.cpp:4-124): the interpolator of
+// voxel i from the fields (component stride ng, voxel strides sy and sz), as
+// five 4-vectors (ex, dexdy, dexdz, d2exdydz; ey ...; ez ...; cbx, dcbxdx,
+// cby, dcbydy; cbz, dcbzdz, 0, 0).
+template <class R, class V4>
+__device__ __forceinline__ void interpolator_at(const R* __restrict__ fields, size_t ng, int sy,
+                                                int sz, int i, V4& o0, V4& o1, V4& o2, V4& o3,
+                                                V4& o4) {
+  const R* ex = fields;
+  const R* ey = fields + ng;
+  const R* ez = fields + 2 * ng;
+  const R* bx = fields + 3 * ng;
+  const R* by = fields + 4 * ng;
+  const R* bz = fields + 5 * ng;
+  const R fourth = 1.0 / 4.0;
+  const R half = 1.0 / 2.0;
+  {
+    const R a = ex[i], b = ex[i + sy], c = ex[i + sz], d = ex[i + sy + sz];
+    o0.x = fourth * ((d + a) + (b + c));
+    o0.y = fourth * ((d - a) + (b - c));
+    o0.z = fourth * ((d - a) - (b - c));
+    o0.w = fourth * ((d + a) - (b + c));
+  }
+  {
+    const R a = ey[i], b = ey[i + sz], c = ey[i + 1], d = ey[i + 1 + sz];
+    o1.x = fourth * ((d + a) + (b + c));
+    o1.y = fourth * ((d - a) + (b - c));
+    o1.z = fourth * ((d - a) - (b - c));
+    o1.w = fourth * ((d + a) - (b + c));
+  }
+  {
+    const R a = ez[i], b = ez[i + 1], c = ez[i + sy], d = ez[i + 1 + sy];
+    o2.x = fourth * ((d + a) + (b + c));
+    o2.y = fourth * ((d - a) + (b - c));
+    o2.z = fourth * ((d - a) - (b - c));
+    o2.w = fourth * ((d + a) - (b + c));
+  }
+  {
+    const R a = bx[i], b = bx[i + 1];
+    o3.x = half * (b + a);
+    o3.y = half * (b - a);
+    const R c = by[i], d = by[i + sy];
+    o3.z = half * (d + c);
+    o3.w = half * (d - c);
+    const R e = bz[i], f = bz[i + sz];
+    o4.x = half * (f + e);
+    o4.y = half * (f - e);
+    o4.z = 0;
+    o4.w = 0;
+  }
+}
+
+// One thread per interior cell, x fastest so neighbouring lanes read
+// neighbouring voxels.
 template <class R>
 __global__ void __launch_bounds__(kBlock)
 k_load_interpolator(const R* __restrict__ fields, R* __restrict__ interp, Grid g) {
@@ -64,50 +116,9 @@ k_load_interpolator(const R* __restrict__ fields, R* __restrict__ interp, Grid g
   const int y = 1 + (t / g.nx) % g.ny;
   const int z = 1 + t / (g.nx * g.ny);
   const int i = voxel(g, x, y, z);
-  const R* ex = fields;
-  const R* ey = fields + (size_t)g.ncells;
-  const R* ez = fields + 2 * (size_t)g.ncells;
-  const R* bx = fields + 3 * (size_t)g.ncells;
-  const R* by = fields + 4 * (size_t)g.ncells;
-  const R* bz = fields + 5 * (size_t)g.ncells;
-  const R fourth = 1.0 / 4.0;
-  const R half = 1.0 / 2.0;
   typedef typename Vec4T<R>::type V4;
   V4 o0, o1, o2, o3, o4;
-  {
-    const R a = ex[i], b = ex[i + g.sy], c = ex[i + g.sz], d = ex[i + g.sy + g.sz];
-    o0.x = fourth * ((d + a) + (b + c));
-    o0.y = fourth * ((d - a) + (b - c));
-    o0.z = fourth * ((d - a) - (b - c));
-    o0.w = fourth * ((d + a) - (b + c));
-  }
-  {
-    const R a = ey[i], b = ey[i + g.sz], c = ey[i + 1], d = ey[i + 1 + g.sz];
-    o1.x = fourth * ((d + a) + (b + c));
-    o1.y = fourth * ((d - a) + (b - c));
-    o1.z = fourth * ((d - a) - (b - c));
-    o1.w = fourth * ((d + a) - (b + c));
-  }
-  {
-    const R a = ez[i], b = ez[i + 1], c = ez[i + g.sy], d = ez[i + 1 + g.sy];
-    o2.x = fourth * ((d + a) + (b + c));
-    o2.y = fourth * ((d - a) + (b - c));
-    o2.z = fourth * ((d - a) - (b - c));
-    o2.w = fourth * ((d + a) - (b + c));
-  }
-  {
-    const R a = bx[i], b = bx[i + 1];
-    o3.x = half * (b + a);
-    o3.y = half * (b - a);
-    const R c = by[i], d = by[i + g.sy];
-    o3.z = half * (d + c);
-    o3.w = half * (d - c);
-    const R e = bz[i], f = bz[i + g.sz];
-    o4.x = half * (f + e);
-    o4.y = half * (f - e);
-    o4.z = 0;
-    o4.w = 0;
-  }
+  interpolator_at(fields, (size_t)g.ncells, g.sy, g.sz, i, o0, o1, o2, o3, o4);
   V4* dst = reinterpret_cast<V4*>(interp + (size_t)i * kInterpStride);
   dst[0] = o0;
   dst[1] = o1;

@@ -498,11 +498,20 @@ __device__ __forceinline__ size_t dep_index(int d, int w, int MW, size_t MP, int
   return ((size_t)d * (size_t)MW + (size_t)w) * MP + (size_t)ci;
 }
 
+// Where pass 1 takes each cell's interpolator from.
+enum Gather : int {
+  kGatherRead = 0,   // the interpolator array (load_interpolator_array ran)
+  kGatherFields = 1,  // computed from the fields, as load_interpolator_array would
+  kGatherStore = 2,   // computed from the fields and also stored in the array
+};
+
 template <class R> struct PushPass {
   ColState<R> cs;
   unsigned long long* dep;  // departure masks written by this pass
   int* dep_words;           // [nc_pad / 64] words written per group
-  const R* interp;
+  R* interp;
+  const R* fields;  // read when gather != kGatherRead
+  int gather;
   R* acc;  // slot-major [12][Ng]
   CellGrid g;
   int ncells;  // Ng
@@ -512,6 +521,31 @@ template <class R> struct PushPass {
   unsigned* movers;
   int* max_count;
 };
+
+// The interpolator of voxel v for pass 1: read from the array, or gathered
+// from the fields with load_interpolator_array's arithmetic (bit-identical),
+// which saves writing the array and reading it back.
+template <class R, class V4>
+__device__ __forceinline__ void cell_interpolator(const PushPass<R>& a, int v, V4& e0, V4& e1,
+                                                  V4& e2, V4& b0, V4& b1) {
+  V4* f = reinterpret_cast<V4*>(a.interp + (size_t)v * kInterpStride);
+  if (a.gather == kGatherRead) {
+    e0 = f[0];
+    e1 = f[1];
+    e2 = f[2];
+    b0 = f[3];
+    b1 = f[4];
+    return;
+  }
+  interpolator_at(a.fields, (size_t)a.ncells, a.g.sy, a.g.sz, v, e0, e1, e2, b0, b1);
+  if (a.gather == kGatherStore) {
+    f[0] = e0;
+    f[1] = e1;
+    f[2] = e2;
+    f[3] = b0;
+    f[4] = b1;
+  }
+}
 
 // Pass 1: push + first streak, one lane per cell (src/push.h:65-295).
 // A particle that stays is stored with its new position.  A particle that
@@ -531,14 +565,7 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
   const int v = valid ? interior_to_voxel(a.g, ci) : 0;
   typedef typename Vec4T<R>::type V4;
   V4 e0 = {}, e1 = {}, e2 = {}, b0 = {}, b1 = {};
-  if (valid) {
-    const V4* f = reinterpret_cast<const V4*>(a.interp + (size_t)v * kInterpStride);
-    e0 = f[0];
-    e1 = f[1];
-    e2 = f[2];
-    b0 = f[3];
-    b1 = f[4];
-  }
+  if (valid) cell_interpolator(a, v, e0, e1, e2, b0, b1);
   const R one = 1.;
   const R one_third = 1. / 3.;
   const R qdt_2mc = a.qdt_2mc;
@@ -837,14 +864,7 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
   const int v = valid ? interior_to_voxel(a.g, ci) : 0;
   typedef typename Vec4T<R>::type V4;
   V4 e0 = {}, e1 = {}, e2 = {}, b0 = {}, b1 = {};
-  if (valid) {
-    const V4* f = reinterpret_cast<const V4*>(a.interp + (size_t)v * kInterpStride);
-    e0 = f[0];
-    e1 = f[1];
-    e2 = f[2];
-    b0 = f[3];
-    b1 = f[4];
-  }
+  if (valid) cell_interpolator(a, v, e0, e1, e2, b0, b1);
   const R one = 1.;
   const R one_third = 1. / 3.;
   const R qdt_2mc = a.qdt_2mc;

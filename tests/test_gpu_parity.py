@@ -223,6 +223,29 @@ def test_one_step_particles_bitwise(dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_fused_step_leaves_last_interpolators(dtype):
+    """The fused step gathers the interpolators inside the push; after a call
+    of n steps the interpolator array holds those of the n-th push, which
+    load_interpolator_array computes from the fields after n - 1 steps.
+    Without particles the step loads them as a stage of its own."""
+    st, _ = states.random_state(7, 5, 4, 6, dtype, seed=83)
+    k = states.step_consts(dtype)
+    with sim_from_state(st) as a, sim_from_state(st) as b:
+        a.step(k, 3, history=False)
+        b.step(k, 2, history=False)
+        b.load_interpolator_array()
+        assert_bitwise(a.download_interpolators(), b.download_interpolators(), "interpolators")
+        assert a.device_errors() == 0 and b.device_errors() == 0
+    st0, _ = states.random_state(7, 5, 4, 1, dtype, seed=84)
+    with Simulation(st0.nx, st0.ny, st0.nz, 1, dtype) as sim:
+        sim.upload_fields(st0.fields)
+        sim.step(k, 1, history=False)
+        got = sim.download_interpolators()
+    st0.load_interpolator()
+    assert_bitwise(got, st0.interp, "interpolators without particles")
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
 def test_many_steps_energy_agreement(dtype):
     """30 fused steps: trajectories drift apart only through the reordered J
     sums; total field energy stays within a small relative band."""
@@ -381,7 +404,8 @@ def test_reflect_boundary_unsupported():
 
 def test_stage_timing_masks():
     """Stage timing counts only the stages asked for: the push alone, then
-    every stage of the fused step once per step."""
+    every stage of the fused step once per step (the interpolator load runs
+    inside the push while particles are resident)."""
     st, _ = states.random_state(4, 3, 3, 6, np.float32, seed=201)
     k = states.step_consts(np.float32)
     with sim_from_state(st) as sim:
@@ -393,9 +417,10 @@ def test_stage_timing_masks():
         sim.set_timing(True)
         sim.step(dict(k, energy_interval=1), 2, history=True)
         t = sim.stage_times()
-        for name in ("interp", "push", "unload", "advance_e", "energy"):
+        for name in ("push", "unload", "advance_e", "energy"):
             assert t[name][0] == 2, name
         assert t["advance_b"][0] == 4
+        assert t["interp"][0] == 0
         sim.set_timing(False)
         sim.step(k, 1, history=False)
         assert all(v[0] == 0 for v in sim.stage_times().values())
