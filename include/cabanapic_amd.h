@@ -240,7 +240,10 @@ int cpic_sort_particles(cpic_context *ctx);
  * receives one row of 3 doubles (E, B, kinetic) per step; entries of steps
  * without the corresponding dump are left untouched.  The energies stay on
  * the device until the loop ends (one synchronisation per call, where the
- * reference synchronises at every dump).  Synchronous at return. */
+ * reference synchronises at every dump).  With particles resident the push
+ * computes each cell's interpolator from the fields itself (the same
+ * values); the interpolator array is written on the last step, so at return
+ * it holds the last push's interpolators.  Synchronous at return. */
 int cpic_step(cpic_context *ctx, const cpic_step_params *sp, int64_t nsteps,
               double *hist);
 
