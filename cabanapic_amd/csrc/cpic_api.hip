@@ -105,7 +105,7 @@ struct cpic_context : ColumnSet {
   long nc_pad = 0;  // interior cells rounded up to whole blocks (kBlock lanes)
   long cap = 0;     // particle capacity declared at creation
   long column_rows = 0;  // requested rows per column (0 = automatic)
-  int* occ_host = nullptr;        // pinned copies of the last push's max_count and error flags
+  int* occ_host = nullptr;        // pinned copies of the last push's error flags and max_count
   hipEvent_t occ_event = nullptr;  // recorded after those copies
   bool occ_pending = false;        // a copy is in flight
   long np = 0;      // particles resident (conserved by the push)
@@ -116,8 +116,14 @@ struct cpic_context : ColumnSet {
   int cur = 0;             // buffer holding the current particles
   long pushes = 0;         // pushes since the last upload (the first of every
                            // compaction interval packs the columns)
+  // device status words, one allocation (status): [0] error flags (err),
+  // [1] largest column occupancy seen by the last push (max_count), [2]
+  // particles that left their cell (movers), [3] overflow records
+  // (ovf_count); a push clears [1..3] with one launch and the host reads
+  // [0..1] back with one copy
+  int* status = nullptr;
   unsigned int* ovf_count = nullptr;
-  int* max_count = nullptr;  // largest column occupancy seen by the last push
+  int* max_count = nullptr;
 
   void* fields = nullptr;
   void* interp = nullptr;
@@ -293,11 +299,10 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
                int compact, int gather = kGatherRead) {
   StageTimer t(c, CPIC_STAGE_PUSH);
   int s;
-  if ((s = fill_zero<unsigned int>(c, c->movers, 1))) return s;
+  // max_count, movers and ovf_count
+  if ((s = fill_zero<int>(c, c->status + 1, 3))) return s;
   if (c->np == 0) return t.finish();
   if (c->D == 0) return fail(CPIC_ERR_INVALID, "particles resident without columns");
-  if ((s = fill_zero<unsigned int>(c, c->ovf_count, 1))) return s;
-  if ((s = fill_zero<int>(c, c->max_count, 1))) return s;
   const int blocks = cdiv(c->nc, kBlock);
   Overflow<R> ovf{(R*)c->ovf_rec, c->ovf_cell, c->ovf_count, c->ovf_cap};
   RecordList<R> L1{(R*)c->rec1, c->rc1, c->M1, (int)c->nc};
@@ -418,10 +423,9 @@ int stage_advance_b(cpic_context* c, R px, R py, R pz) {
   // ES_Field_Solver_1D::advance_b does nothing (src/fields.h:470-480)
   if (c->solver == CPIC_FIELD_SOLVER_ES_1D) return t.finish();
   const long n = c->nx * c->ny * c->nz;
+  // the kernel also writes the B ghosts (the periodic copy)
   k_advance_b<R><<<cdiv(n, kBlock), kBlock, 0, c->stream>>>((R*)c->fields, c->grid, px, py, pz);
   CHECK_LAUNCH();
-  int s = launch_copy_ghosts<R>(c, 3);
-  if (s) return s;
   return t.finish();
 }
 
@@ -445,7 +449,9 @@ int stage_advance_e(cpic_context* c, R px, R py, R pz, R dt_eps0) {
     CHECK_LAUNCH();
     return t.finish();
   }
-  int s = stage_ghosts_j<R>(c);
+  // the J fold, then advance_e, which also writes the J ghosts (the
+  // periodic copy that follows the fold in stage_ghosts_j)
+  int s = launch_fold_j<R>(c);
   if (s) return s;
   const long n = (c->nx + 1) * (c->ny + 1) * (c->nz + 1);
   k_advance_e<R><<<cdiv(n, kBlock), kBlock, 0, c->stream>>>((R*)c->fields, c->grid, px, py, pz,
@@ -603,9 +609,8 @@ int note_occupancy(cpic_context* c) {
     HIP_TRY(hipHostMalloc((void**)&c->occ_host, 2 * sizeof(int), hipHostMallocDefault));
     HIP_TRY(hipEventCreateWithFlags(&c->occ_event, hipEventDisableTiming));
   }
-  HIP_TRY(hipMemcpyAsync(c->occ_host, c->max_count, sizeof(int), hipMemcpyDeviceToHost,
-                         c->stream));
-  HIP_TRY(hipMemcpyAsync(c->occ_host + 1, c->err, sizeof(int), hipMemcpyDeviceToHost,
+  // error flags and max_count, adjacent in the status words
+  HIP_TRY(hipMemcpyAsync(c->occ_host, c->status, 2 * sizeof(int), hipMemcpyDeviceToHost,
                          c->stream));
   HIP_TRY(hipEventRecord(c->occ_event, c->stream));
   c->occ_pending = true;
@@ -640,10 +645,10 @@ int check_headroom(cpic_context* c) {
   if (!c->occ_pending) return CPIC_OK;
   HIP_TRY(hipEventSynchronize(c->occ_event));
   c->occ_pending = false;
-  if (c->occ_host[1])
+  if (c->occ_host[0])
     return fail(CPIC_ERR_DEVICE, "an earlier push failed on the device (flags=%d)",
-                c->occ_host[1]);
-  if (c->D == 0 || c->occ_host[0] < c->D - std::max(8, c->D / 4)) return CPIC_OK;
+                c->occ_host[0]);
+  if (c->D == 0 || c->occ_host[1] < c->D - std::max(8, c->D / 4)) return CPIC_OK;
   const int D = (c->D * 3 / 2 + 63) / 64 * 64;  // 1.5x the rows
   return with_real(c, [&](auto r) { return regrow_columns<decltype(r)>(c, D); });
 }
@@ -871,10 +876,11 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
     if ((s = alloc(&c->acc, c->rsz() * 12 * c->ncells))) return bail(s);
     if ((s = alloc((void**)&c->partials, sizeof(double) * 3 * c->n_partials))) return bail(s);
     if ((s = alloc((void**)&c->red_out, sizeof(double) * 4))) return bail(s);
-    if ((s = alloc((void**)&c->err, sizeof(int)))) return bail(s);
-    if ((s = alloc((void**)&c->movers, sizeof(unsigned int)))) return bail(s);
-    if ((s = alloc((void**)&c->ovf_count, sizeof(unsigned int)))) return bail(s);
-    if ((s = alloc((void**)&c->max_count, sizeof(int)))) return bail(s);
+    if ((s = alloc((void**)&c->status, 4 * sizeof(int)))) return bail(s);
+    c->err = c->status;
+    c->max_count = c->status + 1;
+    c->movers = reinterpret_cast<unsigned int*>(c->status + 2);
+    c->ovf_count = reinterpret_cast<unsigned int*>(c->status + 3);
     s = with_real(c, [&](auto r) {
       using R = decltype(r);
       int t;
@@ -884,8 +890,7 @@ int cpic_create(const cpic_config* cfg, cpic_context** out) {
       return CPIC_OK;
     });
     if (s) return bail(s);
-    if (hipMemsetAsync(c->err, 0, sizeof(int), c->stream) != hipSuccess ||
-        hipMemsetAsync(c->movers, 0, sizeof(unsigned int), c->stream) != hipSuccess ||
+    if (hipMemsetAsync(c->status, 0, 4 * sizeof(int), c->stream) != hipSuccess ||
         hipMemsetAsync(c->red_out, 0, sizeof(double) * 4, c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess)
       return bail(fail(CPIC_ERR_HIP, "context initialisation failed"));
@@ -905,9 +910,8 @@ int cpic_destroy(cpic_context* c) {
     }
     for (auto e : c->free_events) (void)hipEventDestroy(e);
     free_column_set(*c);
-    void* ptrs[] = {c->ovf_count, c->max_count, c->fields,   c->interp,   c->acc,
-                    c->partials,  c->red_out,   c->err,      c->movers,   c->hist_dev,
-                    c->gauss_q,   c->gauss_base, c->gauss_out};
+    void* ptrs[] = {c->status,  c->fields,     c->interp,   c->acc,     c->partials,
+                    c->red_out, c->hist_dev,   c->gauss_q,  c->gauss_base, c->gauss_out};
     for (void* p : ptrs)
       if (p) (void)hipFree(p);
     if (c->occ_event) (void)hipEventDestroy(c->occ_event);

@@ -259,7 +259,35 @@ __global__ void __launch_bounds__(kBlock) k_copy_ghosts(R* __restrict__ fields, 
 }
 
 // ---------------------------------------------------------------------------
-// EM_Field_Solver::advance_b (src/fields.h:668-719), interior cells.
+// The ghost voxels whose periodic source is interior voxel (x, y, z): each
+// coordinate either its own or its ghost image (n + 1 for 1, 0 for n; both
+// when n = 1), other than the voxel itself.  Every ghost voxel is the image
+// of exactly one interior voxel (the one k_copy_ghosts copies it from), so a
+// kernel that computes interior values can write the ghost copies itself.
+template <class F>
+__device__ __forceinline__ void for_each_ghost_image(const Grid& g, int x, int y, int z, F&& f) {
+  if (x != 1 && x != g.nx && y != 1 && y != g.ny && z != 1 && z != g.nz) return;
+  int xs[3] = {x, x, x}, ys[3] = {y, y, y}, zs[3] = {z, z, z};
+  int nxs = 1, nys = 1, nzs = 1;
+  if (x == 1) xs[nxs++] = g.nx + 1;
+  if (x == g.nx) xs[nxs++] = 0;
+  if (y == 1) ys[nys++] = g.ny + 1;
+  if (y == g.ny) ys[nys++] = 0;
+  if (z == 1) zs[nzs++] = g.nz + 1;
+  if (z == g.nz) zs[nzs++] = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int b = 0; b < 3; ++b)
+#pragma unroll
+      for (int a = 0; a < 3; ++a)
+        if (c < nzs && b < nys && a < nxs && (a | b | c)) f(voxel(g, xs[a], ys[b], zs[c]));
+}
+
+// EM_Field_Solver::advance_b (src/fields.h:668-719), interior cells, then
+// the periodic ghost copy of B (serial_update_ghosts_B, src/fields.h:11-100):
+// each thread also writes its voxel's new B into the ghosts it is the source
+// of (no thread reads B ghosts here).
 template <class R>
 __global__ void __launch_bounds__(kBlock) k_advance_b(R* __restrict__ fields, Grid g, R px, R py, R pz) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -276,12 +304,24 @@ __global__ void __launch_bounds__(kBlock) k_advance_b(R* __restrict__ fields, Gr
   R* by = fields + 4 * N;
   R* bz = fields + 5 * N;
   const R e_x = ex[i], e_y = ey[i], e_z = ez[i];
-  bx[i] -= (py * (ez[i + g.sy] - e_z) - pz * (ey[i + g.sz] - e_y));
-  by[i] -= (pz * (ex[i + g.sz] - e_x) - px * (ez[i + 1] - e_z));
-  bz[i] -= (px * (ey[i + 1] - e_y) - py * (ex[i + g.sy] - e_x));
+  const R nbx = bx[i] - (py * (ez[i + g.sy] - e_z) - pz * (ey[i + g.sz] - e_y));
+  const R nby = by[i] - (pz * (ex[i + g.sz] - e_x) - px * (ez[i + 1] - e_z));
+  const R nbz = bz[i] - (px * (ey[i + 1] - e_y) - py * (ex[i + g.sy] - e_x));
+  bx[i] = nbx;
+  by[i] = nby;
+  bz[i] = nbz;
+  for_each_ghost_image(g, x, y, z, [&](int v) {
+    bx[v] = nbx;
+    by[v] = nby;
+    bz[v] = nbz;
+  });
 }
 
-// EM_Field_Solver::advance_e (src/fields.h:618-665), cells [1,n+1]^3.
+// EM_Field_Solver::advance_e (src/fields.h:618-665), cells [1,n+1]^3, after
+// the J fold.  The periodic ghost copy of J (serial_update_ghosts,
+// src/fields.h:11-100) happens here too: a voxel on the high ghost layer
+// reads J from its periodic source (the value the copy gives it), and each
+// interior thread writes its voxel's J into the ghosts it is the source of.
 template <class R>
 __global__ void __launch_bounds__(kBlock)
 k_advance_e(R* __restrict__ fields, Grid g, R px, R py, R pz, R cj) {
@@ -299,13 +339,23 @@ k_advance_e(R* __restrict__ fields, Grid g, R px, R py, R pz, R cj) {
   const R* bx = fields + 3 * N;
   const R* by = fields + 4 * N;
   const R* bz = fields + 5 * N;
-  const R* jx = fields + 6 * N;
-  const R* jy = fields + 7 * N;
-  const R* jz = fields + 8 * N;
+  R* jx = fields + 6 * N;
+  R* jy = fields + 7 * N;
+  R* jz = fields + 8 * N;
+  const bool interior = x <= g.nx && y <= g.ny && z <= g.nz;
+  const int js = interior ? i
+                          : voxel(g, x > g.nx ? 1 : x, y > g.ny ? 1 : y, z > g.nz ? 1 : z);
+  const R j_x = jx[js], j_y = jy[js], j_z = jz[js];
   const R b_x = bx[i], b_y = by[i], b_z = bz[i];
-  ex[i] = ex[i] + (-cj * jx[i]) + (py * (b_z - bz[i - g.sy]) - pz * (b_y - by[i - g.sz]));
-  ey[i] = ey[i] + (-cj * jy[i]) + (pz * (b_x - bx[i - g.sz]) - px * (b_z - bz[i - 1]));
-  ez[i] = ez[i] + (-cj * jz[i]) + (px * (b_y - by[i - 1]) - py * (b_x - bx[i - g.sy]));
+  ex[i] = ex[i] + (-cj * j_x) + (py * (b_z - bz[i - g.sy]) - pz * (b_y - by[i - g.sz]));
+  ey[i] = ey[i] + (-cj * j_y) + (pz * (b_x - bx[i - g.sz]) - px * (b_z - bz[i - 1]));
+  ez[i] = ez[i] + (-cj * j_z) + (px * (b_y - by[i - 1]) - py * (b_x - bx[i - g.sy]));
+  if (interior)
+    for_each_ghost_image(g, x, y, z, [&](int v) {
+      jx[v] = j_x;
+      jy[v] = j_y;
+      jz[v] = j_z;
+    });
 }
 
 // ---------------------------------------------------------------------------
