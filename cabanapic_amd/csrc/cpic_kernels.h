@@ -266,7 +266,13 @@ __global__ void __launch_bounds__(kBlock) k_copy_ghosts(R* __restrict__ fields, 
 // kernel that computes interior values can write the ghost copies itself.
 template <class F>
 __device__ __forceinline__ void for_each_ghost_image(const Grid& g, int x, int y, int z, F&& f) {
-  if (x != 1 && x != g.nx && y != 1 && y != g.ny && z != 1 && z != g.nz) return;
+  const bool yz_inner = y != 1 && y != g.ny && z != 1 && z != g.nz;
+  if (yz_inner) {
+    // the common case: on the x faces only, one image (every wave holds
+    // such a lane)
+    if (g.nx > 1 && (x == 1 || x == g.nx)) f(voxel(g, x == 1 ? g.nx + 1 : 0, y, z));
+    if (g.nx > 1 || x != 1) return;
+  }
   int xs[3] = {x, x, x}, ys[3] = {y, y, y}, zs[3] = {z, z, z};
   int nxs = 1, nys = 1, nzs = 1;
   if (x == 1) xs[nxs++] = g.nx + 1;
