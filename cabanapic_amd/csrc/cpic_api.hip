@@ -95,6 +95,10 @@ struct ColumnSet {
   int ovf_cap = 0;
   unsigned long long* dep = nullptr;  // departure masks of pass 1, [7][MW][nc_pad]
   int* dep_words = nullptr;           // mask words written by pass 1, per group
+  // per group, in dep_words' allocation: set by the first finish pass when
+  // it writes a record addressed to a cell of the group, cleared by the
+  // second, whose waves without a mark have nothing to do
+  int* rec_flags = nullptr;
 };
 
 constexpr int kStatusStride = 64;  // ints per status line (256 bytes)
@@ -378,6 +382,8 @@ int stage_push(cpic_context* c, R qdt_2mc, R cdt_dx, R cdt_dy, R cdt_dz, R qsp, 
     f.ncells = (int)c->ncells;
     f.qsp = qsp;
     f.last = (pass == 1) ? 1 : 0;
+    f.flag_set = (pass == 0) ? c->rec_flags : nullptr;
+    f.flag_skip = (pass == 1) ? c->rec_flags : nullptr;
     f.err = c->err;
     c->compensated ? k_finish_pass<R, true><<<blocks, kBlock, 0, c->stream>>>(f)
                    : k_finish_pass<R, false><<<blocks, kBlock, 0, c->stream>>>(f);
@@ -545,13 +551,14 @@ int alloc_column_set(cpic_context* c, int D, int M1, int M2, ColumnSet& out) {
     if ((t = alloc(&s.ovf_rec, (size_t)record_bytes(c->real_size) * s.ovf_cap))) return t;
     if ((t = alloc((void**)&s.ovf_cell, sizeof(int) * s.ovf_cap))) return t;
     if ((t = alloc((void**)&s.dep, sizeof(unsigned long long) * kDepMasks * words))) return t;
-    if ((t = alloc((void**)&s.dep_words, sizeof(int) * (c->nc_pad / 64)))) return t;
+    if ((t = alloc((void**)&s.dep_words, sizeof(int) * 2 * (c->nc_pad / 64)))) return t;
+    s.rec_flags = s.dep_words + c->nc_pad / 64;
     for (int b = 0; b < 2; ++b) {
       if ((t = fill_zero<int>(c, s.cnt[b], c->nc_pad))) return t;
       if ((t = fill_zero<unsigned long long>(c, s.mask[b], words))) return t;
     }
     if ((t = fill_zero<unsigned long long>(c, s.dep, kDepMasks * words))) return t;
-    if ((t = fill_zero<int>(c, s.dep_words, c->nc_pad / 64))) return t;
+    if ((t = fill_zero<int>(c, s.dep_words, 2 * (c->nc_pad / 64)))) return t;
     return CPIC_OK;
   };
   if (int t = run()) {

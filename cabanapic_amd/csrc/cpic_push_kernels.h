@@ -723,6 +723,8 @@ template <class R> struct FinishPass {
   int ncells;
   R qsp;
   int last;  // 1: no further pass; new crossings go to the overflow list
+  int* flag_set;   // if set: per group, marked when a record is addressed to it
+  int* flag_skip;  // if set: per group; unmarked groups have no records (cleared here)
   int* err;
 };
 
@@ -737,6 +739,12 @@ template <class R, bool K>
 __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   if (__builtin_amdgcn_readfirstlane(*a.err)) return;  // this push already failed
   const int ci = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a.flag_skip) {
+    // no record of the previous pass is addressed to this wave's cells
+    const int gi = __builtin_amdgcn_readfirstlane(ci >> 6);
+    if (__builtin_amdgcn_readfirstlane(a.flag_skip[gi]) == 0) return;
+    if ((threadIdx.x & 63) == 0) a.flag_skip[gi] = 0;
+  }
   const bool valid = ci < a.g.nc;
   const size_t NC = a.g.nc;
   int s0 = 0, s1 = 0, s2 = 0, s3 = 0, s4 = 0, s5 = 0;
@@ -809,6 +817,7 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
         int kk = a.last ? a.out.M : kd[d2];  // last pass: straight to overflow
         put_record(a.out, a.ovf, a.g, ci, d2, kk, pos, rem, ux, uy, uz, w, tag, a.err);
         if (!a.last) kd[d2] = kk;
+        if (a.flag_set) a.flag_set[face_neighbor(a.g, ci, d2) >> 6] = 1;
       }
     };
     for (int i0 = 0; i0 < tmax; i0 += kRecBatch) {
