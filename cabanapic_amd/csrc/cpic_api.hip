@@ -88,8 +88,8 @@ struct ColumnSet {
   int M1 = 0, M2 = 0;
   void* rec1 = nullptr;
   void* rec2 = nullptr;
-  int* rc1 = nullptr;
-  int* rc2 = nullptr;
+  unsigned char* rc1 = nullptr;  // record counts, [6][nc] bytes
+  unsigned char* rc2 = nullptr;
   void* ovf_rec = nullptr;
   int* ovf_cell = nullptr;
   int ovf_cap = 0;
@@ -546,8 +546,8 @@ int alloc_column_set(cpic_context* c, int D, int M1, int M2, ColumnSet& out) {
     }
     if ((t = alloc(&s.rec1, per * M1))) return t;
     if ((t = alloc(&s.rec2, per * M2))) return t;
-    if ((t = alloc((void**)&s.rc1, sizeof(int) * 6 * c->nc))) return t;
-    if ((t = alloc((void**)&s.rc2, sizeof(int) * 6 * c->nc))) return t;
+    if ((t = alloc((void**)&s.rc1, 6 * (size_t)c->nc))) return t;
+    if ((t = alloc((void**)&s.rc2, 6 * (size_t)c->nc))) return t;
     if ((t = alloc(&s.ovf_rec, (size_t)record_bytes(c->real_size) * s.ovf_cap))) return t;
     if ((t = alloc((void**)&s.ovf_cell, sizeof(int) * s.ovf_cap))) return t;
     if ((t = alloc((void**)&s.dep, sizeof(unsigned long long) * kDepMasks * words))) return t;
@@ -1007,7 +1007,7 @@ int upload_particles(cpic_context* c, int64_t n, const void* x, const void* y, c
   }
   const int D = column_rows_for(c, maxc, n);
   auto up8 = [](long v) { return (int)((v + 7) / 8 * 8); };
-  const int M1 = std::min(D, std::max(16, up8(maxc / 2)));
+  const int M1 = std::min({D, std::max(16, up8(maxc / 2)), kMaxRecords});
   const int M2 = std::min(D, 8);
   for (int64_t p = 0; p < n; ++p) {
     const long long ci = slot[p] >> 32, r = slot[p] & 0xffffffffll;

@@ -425,9 +425,11 @@ template <class R> struct RowCursor {
 
 // Record list k-th slot of (cell, d): [d][k][cell], so the records that
 // neighbouring lanes write or read for the same (d, k) are adjacent.
+constexpr int kMaxRecords = 248;  // record slots per (face, cell): counts fit a byte
+
 template <class R> struct RecordList {
-  R* rec;    // [6][M][Nc] records of RecordLen<R> reals
-  int* cnt;  // [6][Nc]
+  R* rec;              // [6][M][Nc] records of RecordLen<R> reals
+  unsigned char* cnt;  // [6][Nc] records per (face, cell), at most M <= kMaxRecords
   int M;
   int nc;
 };
@@ -846,7 +848,8 @@ __global__ void __launch_bounds__(kBlock) k_finish_pass(FinishPass<R> a) {
   }
   if (valid && !a.last) {
 #pragma unroll
-    for (int dd = 0; dd < 6; ++dd) a.out.cnt[(size_t)dd * NC + ci] = min(kd[dd], a.out.M);
+    for (int dd = 0; dd < 6; ++dd)
+      a.out.cnt[(size_t)dd * NC + ci] = (unsigned char)min(kd[dd], a.out.M);
   }
 }
 
@@ -1193,7 +1196,8 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
       if (own) a.cs.mask_out[(size_t)w * MP + ci] &= ~own;
     }
 #pragma unroll
-    for (int dd = 0; dd < 6; ++dd) a.out.cnt[(size_t)dd * NC + ci] = min(kd[dd], a.out.M);
+    for (int dd = 0; dd < 6; ++dd)
+      a.out.cnt[(size_t)dd * NC + ci] = (unsigned char)min(kd[dd], a.out.M);
   }
 }
 
