@@ -699,10 +699,13 @@ __global__ void __launch_bounds__(kBlock) k_push_pass(PushPass<R> a) {
       for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc.total(k);
     }
   }
-  // the remaining mask and departure words of the column
+  // the remaining mask and departure words of the column: every lane writes
+  // the group's departure words (those of its rows past its own end are
+  // zero), the only ones pass 2 reads
   flush_dep();
   for (int w = 0; w < MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = low_bits(ns - 64 * w);
-  for (int w = cw + 1; w < MW; ++w)
+  const int gw = min((lmax + 63) >> 6, MW);
+  for (int w = cw + 1; w < gw; ++w)
     for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
   a.cs.cnt_out[ci] = ns;
   const int wm = wave_sum_int(nmov);
@@ -1023,10 +1026,7 @@ __global__ void __launch_bounds__(kBlock) k_push_rows(PushPass<R> a) {
       for (int k = 0; k < 12; ++k) a.acc[k * N + v] = acc.total(k);
     }
   }
-  for (int w = (lmax + 63) >> 6; w < MW; ++w) {
-    a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
-    for (int d = 0; d < kDepMasks; ++d) a.dep[dep_index(d, w, MW, MP, ci)] = 0ull;
-  }
+  for (int w = (lmax + 63) >> 6; w < MW; ++w) a.cs.mask_out[(size_t)w * MP + ci] = 0ull;
   a.cs.cnt_out[ci] = hi;
   const int wm = wave_sum_int(nmov);
   if ((threadIdx.x & 63) == 0) {
@@ -1092,23 +1092,29 @@ __global__ void __launch_bounds__(kBlock) k_arrive_pass(ArrivePass<R> a) {
   const R* dummy = a.cs.out;
 
   // words of the departure masks to read: those pass 1 wrote for the
-  // source groups (usually one)
+  // source groups (usually one).  Pass 1 writes only the words of its
+  // group's rows, so a word at or past its group's count is no departure.
   int nw = 0;
+  int ws0 = 0, ws1 = 0, ws2 = 0, ws3 = 0, ws4 = 0, ws5 = 0;
   if (valid) {
-    nw = max(max(a.dep_words[s0 >> 6], a.dep_words[s1 >> 6]),
-             max(max(a.dep_words[s2 >> 6], a.dep_words[s3 >> 6]),
-                 max(a.dep_words[s4 >> 6], a.dep_words[s5 >> 6])));
+    ws0 = a.dep_words[s0 >> 6];
+    ws1 = a.dep_words[s1 >> 6];
+    ws2 = a.dep_words[s2 >> 6];
+    ws3 = a.dep_words[s3 >> 6];
+    ws4 = a.dep_words[s4 >> 6];
+    ws5 = a.dep_words[s5 >> 6];
+    nw = max(max(ws0, ws1), max(max(ws2, ws3), max(ws4, ws5)));
   }
   nw = min(wave_max_int(nw), MW);
   for (int w = 0; w < nw; ++w) {
     unsigned long long d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0;
     if (valid) {
-      d0 = a.dep[dep_index(0, w, MW, MP, s0)];
-      d1 = a.dep[dep_index(1, w, MW, MP, s1)];
-      d2 = a.dep[dep_index(2, w, MW, MP, s2)];
-      d3 = a.dep[dep_index(3, w, MW, MP, s3)];
-      d4 = a.dep[dep_index(4, w, MW, MP, s4)];
-      d5 = a.dep[dep_index(5, w, MW, MP, s5)];
+      if (w < ws0) d0 = a.dep[dep_index(0, w, MW, MP, s0)];
+      if (w < ws1) d1 = a.dep[dep_index(1, w, MW, MP, s1)];
+      if (w < ws2) d2 = a.dep[dep_index(2, w, MW, MP, s2)];
+      if (w < ws3) d3 = a.dep[dep_index(3, w, MW, MP, s3)];
+      if (w < ws4) d4 = a.dep[dep_index(4, w, MW, MP, s4)];
+      if (w < ws5) d5 = a.dep[dep_index(5, w, MW, MP, s5)];
     }
     const int total = __popcll(d0) + __popcll(d1) + __popcll(d2) + __popcll(d3) +
                       __popcll(d4) + __popcll(d5);
