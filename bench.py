@@ -335,7 +335,8 @@ def cpu_baseline(deck, nsteps, nthreads):
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=64,
+                    help="timed steps (64: two full compaction intervals of 32 pushes)")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=128, help="cells per axis")
     ap.add_argument("--nppc", type=int, default=32)
