@@ -383,6 +383,45 @@ def test_crossing_record_overflow_path(dtype):
     _acc_within_reordering_bound(acc, st, absacc, dtype)
 
 
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_third_crossing_records_past_the_byte_cap(dtype):
+    """1200 particles near one corner of a cell move along the diagonal and
+    cross all three faces: the third crossings of some (cell, face) outnumber
+    the 248 record slots a byte count allows (the list holds min(248, ...)
+    slots), and the excess takes the overflow list and the spill fallback,
+    with identical results."""
+    nx, ny, nz = 4, 4, 4
+    n = 1200
+    st = oracle_lib.OracleState(nx, ny, nz, n, dtype)
+    rng = np.random.default_rng(11)
+    X, Y = nx + 2, ny + 2
+    st.cell[:] = 2 + X * (2 + Y * 2)  # interior cell (2, 2, 2)
+    for m in ("x", "y", "z"):
+        st.p[m][:] = rng.uniform(0.9, 0.99, n).astype(dtype)
+    for m in ("ux", "uy", "uz"):
+        st.p[m][:] = rng.uniform(1.9, 2.1, n).astype(dtype)
+    st.p["w"][:] = dtype(1)
+    k = dict(qdt_2mc=dtype(-0.01), cdt_dx=dtype(0.45), cdt_dy=dtype(0.45), cdt_dz=dtype(0.45),
+             qsp=dtype(-1.0))
+    st.load_interpolator()
+    absacc = np.zeros(st.acc.size)
+    st.clear_accumulator()
+    with sim_from_state(st) as sim:
+        sim.load_interpolator_array()
+        sim.clear_accumulator_array()
+        sim.push(**k)
+        st.push(**k, abs_acc=absacc)
+        got = sim.download_particles()
+        assert_particles_bitwise(got, st)
+        acc = sim.download_accumulators()
+        assert sim.device_errors() == 0
+    # particles that crossed all three faces end in cell (3, 3, 3): at least
+    # a third of them share one (cell, face) of their third crossing
+    diag = 3 + X * (3 + Y * 3)
+    assert np.count_nonzero(st.cell == diag) >= 3 * 249
+    _acc_within_reordering_bound(acc, st, absacc, dtype)
+
+
 def test_invalid_cell_rejected_on_upload():
     st, _ = states.random_state(4, 4, 4, 2, np.float32, seed=1)
     sim = Simulation(4, 4, 4, st.np, np.float32)
