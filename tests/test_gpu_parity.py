@@ -246,6 +246,33 @@ def test_fused_step_leaves_last_interpolators(dtype):
 
 
 @pytest.mark.parametrize("dtype", DTYPES, ids=_name)
+def test_fused_step_equals_stage_calls(dtype):
+    """cpic_step (interpolators gathered inside the push, ghost copies written
+    by the Yee kernels, no accumulator clear) leaves the same particles,
+    fields, accumulators and interpolators, bit for bit, as the reference's
+    loop body made of the stage calls (example/example.cpp:216-266)."""
+    st, _ = states.random_state(6, 5, 4, 8, dtype, seed=87, umax=0.6)
+    k = states.step_consts(dtype)
+    with sim_from_state(st) as a, sim_from_state(st) as b:
+        a.step(k, 3, history=False)
+        for _ in range(3):
+            b.load_interpolator_array()
+            b.clear_accumulator_array()
+            b.push(k["qdt_2mc"], k["cdt_dx"], k["cdt_dy"], k["cdt_dz"], k["qsp"])
+            b.unload_accumulator_array(k["dx"], k["dy"], k["dz"], k["dt"])
+            b.advance_b(0.5 * k["px"], 0.5 * k["py"], 0.5 * k["pz"])
+            b.advance_e(k["px"], k["py"], k["pz"], k["dt_eps0"])
+            b.advance_b(0.5 * k["px"], 0.5 * k["py"], 0.5 * k["pz"])
+        ga, gb = states.canonical(a.download_particles()), states.canonical(b.download_particles())
+        for m in list(states.MEMBERS) + ["cell"]:
+            assert_bitwise(ga[m], gb[m], m)
+        assert_bitwise(a.download_fields(), b.download_fields(), "fields")
+        assert_bitwise(a.download_accumulators(), b.download_accumulators(), "accumulators")
+        assert_bitwise(a.download_interpolators(), b.download_interpolators(), "interpolators")
+        assert a.device_errors() == 0 and b.device_errors() == 0
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=_name)
 def test_many_steps_energy_agreement(dtype):
     """30 fused steps: trajectories drift apart only through the reordered J
     sums; total field energy stays within a small relative band."""
