@@ -97,8 +97,30 @@ template <class R> struct TangentPush {
   unsigned* movers;
 };
 
+// A block's accumulator window in LDS: kTanWindow consecutive voxels from
+// base (one before the voxel of the block's first particle; particles are
+// sorted by voxel, so the block's own cells and their x neighbours fall in
+// it).  Deposits into the window are LDS atomics, flushed to the
+// accumulators once per block; others are global atomics.
+constexpr int kTanWindow = 64;
+template <class R> struct TanWindow {
+  int base;
+  R (*v)[kTanWindow];       // [12][kTanWindow] primal
+  double (*t)[kTanWindow];  // [12][kTanWindow] tangent
+};
+
 template <class R>
-__device__ __forceinline__ void deposit_dual(const TangentPush<R>& a, int v, const Dual<R> o[12]) {
+__device__ __forceinline__ void deposit_dual(const TangentPush<R>& a, const TanWindow<R>& w, int v,
+                                             const Dual<R> o[12]) {
+  const unsigned c = (unsigned)(v - w.base);
+  if (c < (unsigned)kTanWindow) {
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      atomicAdd(&w.v[k][c], o[k].v);
+      atomicAdd(&w.t[k][c], o[k].t);
+    }
+    return;
+  }
   const size_t N = a.ncells;
 #pragma unroll
   for (int k = 0; k < 12; ++k) {
@@ -114,7 +136,8 @@ __device__ __forceinline__ void deposit_dual(const TangentPush<R>& a, int v, con
 // consecutive lanes thus cost one atomic per value instead of one per lane.
 // Every lane of the wave must call it.
 template <class R>
-__device__ __forceinline__ void deposit_runs(const TangentPush<R>& a, int key, Dual<R> o[12]) {
+__device__ __forceinline__ void deposit_runs(const TangentPush<R>& a, const TanWindow<R>& w,
+                                             int key, Dual<R> o[12]) {
   const int lane = threadIdx.x & 63;
   const int before = __shfl_up(key, 1, 64);
   int head = (lane == 0 || before != key) ? 1 : 0;
@@ -134,13 +157,14 @@ __device__ __forceinline__ void deposit_runs(const TangentPush<R>& a, int key, D
     if (lane >= off) head |= h;
   }
   const int after = __shfl_down(key, 1, 64);
-  if (key >= 0 && (lane == 63 || after != key)) deposit_dual(a, key, o);
+  if (key >= 0 && (lane == 63 || after != key)) deposit_dual(a, w, key, o);
 }
 
 // The push of particle p (nothing for p >= n); returns the voxel of its own
 // cell, whose deposits are left in own[], or -1.
 template <class R>
-__device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a, int p,
+__device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a,
+                                                     const TanWindow<R>& win, int p,
                                                      Dual<R> own[12]) {
   if (p >= a.n) return -1;
   typedef Dual<R> D;
@@ -212,7 +236,7 @@ __device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a, in
       ci = face_neighbor(a.g, ci, d);
       D o[12];
       d = streak(pos, rem, q, o);
-      deposit_dual(a, interior_to_voxel(a.g, ci), o);
+      deposit_dual(a, win, interior_to_voxel(a.g, ci), o);
     }
     if (d >= 0) atomicOr(a.err, kErrStreakLimit);
   }
@@ -280,11 +304,33 @@ template <class R>
 __global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   typedef Dual<R> D;
+  __shared__ R wv[12][kTanWindow];
+  __shared__ double wt[12][kTanWindow];
+  const int p0 = blockIdx.x * blockDim.x;
+  TanWindow<R> win{(p0 < a.n ? a.src_cell[p0] : 0) - 1, wv, wt};
+  for (int i = threadIdx.x; i < 12 * kTanWindow; i += blockDim.x) {
+    wv[i / kTanWindow][i % kTanWindow] = 0;
+    wt[i / kTanWindow][i % kTanWindow] = 0;
+  }
+  __syncthreads();
   D own[12];  // deposits into the particle's own cell
   // every lane takes part in the wave's run sums, so a lane without a
   // particle (or with an invalid cell) keeps key -1 and no deposit
-  const int key = push_tangent_particle(a, p, own);
-  deposit_runs(a, key, own);
+  const int key = push_tangent_particle(a, win, p, own);
+  deposit_runs(a, win, key, own);
+  __syncthreads();
+  // the window's sums, one global atomic per value that received any
+  const size_t N = a.ncells;
+  for (int i = threadIdx.x; i < 12 * kTanWindow; i += blockDim.x) {
+    const int k = i / kTanWindow, c = i % kTanWindow;
+    const int v = win.base + c;
+    const R x = wv[k][c];
+    const double y = wt[k][c];
+    if ((x != R(0) || y != 0.) && v >= 0 && v < a.ncells) {
+      unsafeAtomicAdd(a.acc + k * N + v, x);
+      unsafeAtomicAdd(a.tacc + k * N + v, y);
+    }
+  }
 }
 
 // Appends the step's row (E, B, dE, dB) to the history at the row a
