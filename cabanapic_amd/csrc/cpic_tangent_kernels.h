@@ -257,17 +257,39 @@ __device__ __forceinline__ int push_tangent_particle(const TangentPush<R>& a,
   return v0;
 }
 
-// Counting sort of the particles by voxel before each tangent push, so that
+// Runs of equal keys in consecutive lanes of the wave: for a lane with
+// key >= 0, the first lane of its run and the run's length.  The particles
+// come mostly sorted by voxel (sorted a few steps ago), so a run stands for
+// many particles of one voxel and takes one atomic instead of one per lane.
+__device__ __forceinline__ void wave_key_run(int key, int& first, int& len) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  const bool head = key >= 0 && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long bounds = __ballot(head || key < 0);
+  const unsigned long long upto = lane == 63 ? heads : heads & ((2ull << lane) - 1);
+  first = upto ? 63 - __clzll(upto) : lane;
+  const unsigned long long after = lane == 63 ? 0ull : bounds & ~((2ull << lane) - 1);
+  const int next = after ? __ffsll((long long)after) - 1 : 64;
+  len = next - first;
+}
+
+// Counting sort of the particles by voxel before a tangent push, so that
 // the run sums of the push's own-cell deposits see a cell's particles in
 // consecutive lanes (particles keep drifting out of the order of their
 // upload).  Count: particles per voxel (an invalid voxel counts in bin 0
-// and is flagged by the push).
+// and is flagged by the push), one atomic per run of a voxel in a wave.
 __global__ void __launch_bounds__(kBlock)
 k_tangent_bin_count(const int* __restrict__ cell, int n, int ncells, int* __restrict__ bins) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int v = cell[p];
-  atomicAdd(bins + ((v >= 0 && v < ncells) ? v : 0), 1);
+  int key = -1;
+  if (p < n) {
+    const int v = cell[p];
+    key = (v >= 0 && v < ncells) ? v : 0;
+  }
+  int first, len;
+  wave_key_run(key, first, len);
+  if (key >= 0 && (int)(threadIdx.x & 63) == first) atomicAdd(bins + key, len);
 }
 
 // Scatter: every particle to the next free position of its voxel's range
@@ -281,9 +303,21 @@ k_tangent_bin_scatter(const R* __restrict__ part, const int* __restrict__ cell,
                       int* __restrict__ cell_out, double* __restrict__ tpart_out,
                       int* __restrict__ tag_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const int v = cell[p];
-  const int o = atomicAdd(start + ((v >= 0 && v < ncells) ? v : 0), 1);
+  int key = -1, v = 0;
+  if (p < n) {
+    v = cell[p];
+    key = (v >= 0 && v < ncells) ? v : 0;
+  }
+  // one atomic per run of a voxel in the wave; the run's lanes take
+  // consecutive positions from its first lane's
+  int first, len;
+  wave_key_run(key, first, len);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (key >= 0 && lane == first) base = atomicAdd(start + key, len);
+  base = __shfl(base, first, 64);
+  if (key < 0) return;
+  const int o = base + (lane - first);
 #pragma unroll
   for (int k = 0; k < 7; ++k) part_out[k * stride + o] = part[k * stride + p];
 #pragma unroll
