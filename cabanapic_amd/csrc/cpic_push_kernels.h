@@ -298,13 +298,15 @@ __device__ __forceinline__ void add12(LaneAcc<R, K>& acc, const R o[12]) {
 template <class R>
 __device__ __forceinline__ int streak_geometry(R pos[3], R rem[3], R q, R dsp[3], R mid[3],
                                                R& v5) {
-  R dir[3], len3[3];
+  R len3[3];
+  int up = 0;  // bit k: the track moves towards +k
 #pragma unroll
   for (int k = 0; k < 3; ++k) {
     dsp[k] = rem[k];
     mid[k] = pos[k];
-    dir[k] = (dsp[k] > 0) ? R(1) : R(-1);
-    len3[k] = (dsp[k] == 0) ? R(3.4e38) : (dir[k] - mid[k]) / dsp[k];
+    const R dir = (dsp[k] > 0) ? R(1) : R(-1);
+    up |= (dsp[k] > 0) ? 1 << k : 0;
+    len3[k] = (dsp[k] == 0) ? R(3.4e38) : (dir - mid[k]) / dsp[k];
   }
   R len = 2;
   int axis = 3;
@@ -324,7 +326,9 @@ __device__ __forceinline__ int streak_geometry(R pos[3], R rem[3], R q, R dsp[3]
     pos[k] += dsp[k] + dsp[k];
   }
   if (axis == 3) return -1;
-  const R side = dir[axis];
+  // the direction as a bit, not an array read at a run-time index, which
+  // would put the array in scratch memory for the dual-number type
+  const R side = ((up >> axis) & 1) ? R(1) : R(-1);
   if (axis == 0) pos[0] = -side;
   else if (axis == 1) pos[1] = -side;
   else pos[2] = -side;

@@ -333,9 +333,12 @@ k_tangent_bin_scatter(const R* __restrict__ part, const int* __restrict__ cell,
 // summed over the wave's runs of lanes in one cell first; the later streaks
 // of a particle that leaves are added directly.  The primal positions,
 // momenta and cells equal those of the column push; the primal accumulators
-// equal them up to the order of the floating-point sums.
+// equal them up to the order of the floating-point sums.  The FP32 kernel is
+// held to 128 registers (4 waves per SIMD instead of 3; a few spills), which
+// pays for this latency-bound kernel; the FP64 one would spill too much.
 template <class R>
-__global__ void __launch_bounds__(kBlock) k_push_tangent(TangentPush<R> a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1)))
+k_push_tangent(TangentPush<R> a) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   typedef Dual<R> D;
   __shared__ R wv[12][kTanWindow];
