@@ -114,10 +114,14 @@ __device__ __forceinline__ void deposit_dual(const TangentPush<R>& a, const TanW
                                              const Dual<R> o[12]) {
   const unsigned c = (unsigned)(v - w.base);
   if (c < (unsigned)kTanWindow) {
+    // workgroup scope: besides being all that LDS needs, it keeps these
+    // atomics distinct from the global ones below, which the compiler would
+    // otherwise merge with them into one flat atomic (counted by lgkmcnt, so
+    // that every later LDS wait also waits for a global atomic)
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      atomicAdd(&w.v[k][c], o[k].v);
-      atomicAdd(&w.t[k][c], o[k].t);
+      __hip_atomic_fetch_add(&w.v[k][c], o[k].v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      __hip_atomic_fetch_add(&w.t[k][c], o[k].t, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     return;
   }
@@ -334,8 +338,8 @@ k_tangent_bin_scatter(const R* __restrict__ part, const int* __restrict__ cell,
 // of a particle that leaves are added directly.  The primal positions,
 // momenta and cells equal those of the column push; the primal accumulators
 // equal them up to the order of the floating-point sums.  The FP32 kernel is
-// held to 128 registers (4 waves per SIMD instead of 3; a few spills), which
-// pays for this latency-bound kernel; the FP64 one would spill too much.
+// held to 128 registers (4 waves per SIMD instead of 3), which pays for this
+// latency-bound kernel; the FP64 one would spill too much.
 template <class R>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 4 : 1)))
 k_push_tangent(TangentPush<R> a) {
