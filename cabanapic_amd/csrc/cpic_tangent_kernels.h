@@ -282,9 +282,11 @@ __device__ __forceinline__ void wave_key_run(int key, int& first, int& len) {
 // the run sums of the push's own-cell deposits see a cell's particles in
 // consecutive lanes (particles keep drifting out of the order of their
 // upload).  Count: particles per voxel (an invalid voxel counts in bin 0
-// and is flagged by the push), one atomic per run of a voxel in a wave.
+// and is flagged by the push), one atomic per run of a voxel in a wave,
+// whose result gives every particle its rank within its voxel.
 __global__ void __launch_bounds__(kBlock)
-k_tangent_bin_count(const int* __restrict__ cell, int n, int ncells, int* __restrict__ bins) {
+k_tangent_bin_count(const int* __restrict__ cell, int n, int ncells, int* __restrict__ bins,
+                    int* __restrict__ rank) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   int key = -1;
   if (p < n) {
@@ -293,35 +295,29 @@ k_tangent_bin_count(const int* __restrict__ cell, int n, int ncells, int* __rest
   }
   int first, len;
   wave_key_run(key, first, len);
-  if (key >= 0 && (int)(threadIdx.x & 63) == first) atomicAdd(bins + key, len);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (key >= 0 && lane == first) base = atomicAdd(bins + key, len);
+  base = __shfl(base, first, 64);
+  if (key >= 0) rank[p] = base + (lane - first);
 }
 
-// Scatter: every particle to the next free position of its voxel's range
-// (start: the exclusive scan of the counts, advanced by this kernel); the
-// order within a voxel is unspecified.
+// Scatter: every particle to its voxel's range (start: the exclusive scan
+// of the counts) at its rank; the order within a voxel is the order in
+// which the count's atomics arrived.
 template <class R>
 __global__ void __launch_bounds__(kBlock)
 k_tangent_bin_scatter(const R* __restrict__ part, const int* __restrict__ cell,
                       const double* __restrict__ tpart, const int* __restrict__ tag, int n,
-                      long stride, int ncells, int* __restrict__ start, R* __restrict__ part_out,
+                      long stride, int ncells, const int* __restrict__ start,
+                      const int* __restrict__ rank, R* __restrict__ part_out,
                       int* __restrict__ cell_out, double* __restrict__ tpart_out,
                       int* __restrict__ tag_out) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  int key = -1, v = 0;
-  if (p < n) {
-    v = cell[p];
-    key = (v >= 0 && v < ncells) ? v : 0;
-  }
-  // one atomic per run of a voxel in the wave; the run's lanes take
-  // consecutive positions from its first lane's
-  int first, len;
-  wave_key_run(key, first, len);
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (key >= 0 && lane == first) base = atomicAdd(start + key, len);
-  base = __shfl(base, first, 64);
-  if (key < 0) return;
-  const int o = base + (lane - first);
+  if (p >= n) return;
+  const int v = cell[p];
+  const int key = (v >= 0 && v < ncells) ? v : 0;
+  const int o = start[key] + rank[p];
 #pragma unroll
   for (int k = 0; k < 7; ++k) part_out[k * stride + o] = part[k * stride + p];
 #pragma unroll
