@@ -100,19 +100,54 @@ template <class R> struct TangentPush {
 // A block's accumulator window in LDS: kTanWindow consecutive voxels from
 // base (one before the voxel of the block's first particle; particles are
 // sorted by voxel, so the block's own cells and their x neighbours fall in
-// it).  Deposits into the window are LDS atomics, flushed to the
-// accumulators once per block; others are global atomics.
+// it), and four side segments of kTanSide voxels each, the same run shifted
+// by +y, -y, +z and -z (the neighbours that crossing particles enter, and
+// the cells of particles that left the block's cells since the last sort).
+// Deposits into the window are LDS atomics, flushed to the accumulators
+// once per block with one atomic per value that received any (the flush's
+// lanes take consecutive voxels, so the L2 passes them on to memory a line
+// at a time); others are global atomics.
 constexpr int kTanWindow = 64;
+constexpr int kTanSide = 32;
 template <class R> struct TanWindow {
   int base;
-  R (*v)[kTanWindow];       // [12][kTanWindow] primal
-  double (*t)[kTanWindow];  // [12][kTanWindow] tangent
+  int sy, sz;                      // voxel strides in y and z
+  R (*v)[kTanWindow];              // [12][kTanWindow] primal
+  double (*t)[kTanWindow];         // [12][kTanWindow] tangent
+  R (*sv)[4][kTanSide];            // [12][4][kTanSide] primal, side segments
+  double (*st)[4][kTanSide];       // [12][4][kTanSide] tangent, side segments
 };
+
+// offset of side segment s (0..3: +y, -y, +z, -z) from the window's base
+template <class R> __device__ __forceinline__ int side_offset(const TanWindow<R>& w, int s) {
+  return s == 0 ? w.sy : s == 1 ? -w.sy : s == 2 ? w.sz : -w.sz;
+}
 
 template <class R>
 __device__ __forceinline__ void deposit_dual(const TangentPush<R>& a, const TanWindow<R>& w, int v,
                                              const Dual<R> o[12]) {
-  const unsigned c = (unsigned)(v - w.base);
+  const int r = v - w.base;
+  const unsigned c = (unsigned)r;
+  // the first segment that holds v: the run, then the side segments
+  int seg = -1;
+  unsigned cs = 0;
+  if (c >= (unsigned)kTanWindow) {
+#pragma unroll
+    for (int s2 = 3; s2 >= 0; --s2) {
+      const unsigned d = (unsigned)(r - side_offset(w, s2));
+      if (d < (unsigned)kTanSide) seg = s2, cs = d;
+    }
+    if (seg >= 0) {
+#pragma unroll
+      for (int k = 0; k < 12; ++k) {
+        __hip_atomic_fetch_add(&w.sv[k][seg][cs], o[k].v, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        __hip_atomic_fetch_add(&w.st[k][seg][cs], o[k].t, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      return;
+    }
+  }
   if (c < (unsigned)kTanWindow) {
     // workgroup scope: besides being all that LDS needs, it keeps these
     // atomics distinct from the global ones below, which the compiler would
@@ -343,11 +378,18 @@ k_push_tangent(TangentPush<R> a) {
   typedef Dual<R> D;
   __shared__ R wv[12][kTanWindow];
   __shared__ double wt[12][kTanWindow];
+  __shared__ R sv[12][4][kTanSide];
+  __shared__ double st[12][4][kTanSide];
   const int p0 = blockIdx.x * blockDim.x;
-  TanWindow<R> win{(p0 < a.n ? a.src_cell[p0] : 0) - 1, wv, wt};
+  TanWindow<R> win{(p0 < a.n ? a.src_cell[p0] : 0) - 1, a.g.sy, a.g.sz, wv, wt, sv, st};
   for (int i = threadIdx.x; i < 12 * kTanWindow; i += blockDim.x) {
     wv[i / kTanWindow][i % kTanWindow] = 0;
     wt[i / kTanWindow][i % kTanWindow] = 0;
+  }
+  for (int i = threadIdx.x; i < 12 * 4 * kTanSide; i += blockDim.x) {
+    const int k = i / (4 * kTanSide), sg = (i / kTanSide) % 4, c = i % kTanSide;
+    sv[k][sg][c] = 0;
+    st[k][sg][c] = 0;
   }
   __syncthreads();
   D own[12];  // deposits into the particle's own cell
@@ -363,6 +405,16 @@ k_push_tangent(TangentPush<R> a) {
     const int v = win.base + c;
     const R x = wv[k][c];
     const double y = wt[k][c];
+    if ((x != R(0) || y != 0.) && v >= 0 && v < a.ncells) {
+      unsafeAtomicAdd(a.acc + k * N + v, x);
+      unsafeAtomicAdd(a.tacc + k * N + v, y);
+    }
+  }
+  for (int i = threadIdx.x; i < 12 * 4 * kTanSide; i += blockDim.x) {
+    const int k = i / (4 * kTanSide), sg = (i / kTanSide) % 4, c = i % kTanSide;
+    const int v = win.base + side_offset(win, sg) + c;
+    const R x = sv[k][sg][c];
+    const double y = st[k][sg][c];
     if ((x != R(0) || y != 0.) && v >= 0 && v < a.ncells) {
       unsafeAtomicAdd(a.acc + k * N + v, x);
       unsafeAtomicAdd(a.tacc + k * N + v, y);
