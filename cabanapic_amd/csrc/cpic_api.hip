@@ -571,8 +571,13 @@ int alloc_column_set(cpic_context* c, int D, int M1, int M2, ColumnSet& out) {
 }
 
 // Replaces the context's column set by s (after the stream has drained).
+// If the stream reports an error, s is freed and the current set is kept.
 int install_column_set(cpic_context* c, ColumnSet& s) {
-  HIP_TRY(hipStreamSynchronize(c->stream));
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) {
+    free_column_set(s);
+    return fail(CPIC_ERR_HIP, "column set installation: %s", hipGetErrorString(e));
+  }
   ColumnSet& mine = *c;
   free_column_set(mine);
   mine = s;
