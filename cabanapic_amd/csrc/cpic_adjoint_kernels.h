@@ -106,7 +106,13 @@ k_scan_add(int* __restrict__ out, int n, const int* __restrict__ sums) {
 
 // The context's columns into a snapshot: lane per cell, each occupied row
 // (in row order) copied as its whole slot to slots[first[ci] + k]; cell[]
-// receives the interior cell index of every snapshot slot.
+// receives the interior cell index of every snapshot slot.  Rows are taken
+// kSnapRows at a time: the batch's loads are all issued before its stores,
+// and the batch's slots land next to each other, so each lane completes
+// its output lines in a few consecutive stores.
+constexpr int kSnapRows = 4;
+static_assert(64 % kSnapRows == 0, "a mask word holds whole batches of rows");
+
 template <class R>
 __global__ void __launch_bounds__(kBlock)
 k_snapshot_columns(const R* __restrict__ cols, const int* __restrict__ cnt,
@@ -119,15 +125,31 @@ k_snapshot_columns(const R* __restrict__ cols, const int* __restrict__ cnt,
   long o = first[ci];
   const long end = first[ci + 1];
   unsigned long long m = 0;
-  for (int j = 0; j < len && o < end; ++j) {
-    if ((j & 63) == 0) m = mask[(size_t)(j >> 6) * nc_pad + ci];
-    if (!((m >> (j & 63)) & 1ull)) continue;
-    const V4* s = reinterpret_cast<const V4*>(cols + col_slot(ci, j, D) * kSlotLen);
-    V4* d = reinterpret_cast<V4*>(slots + (size_t)o * kSlotLen);
-    d[0] = s[0];
-    d[1] = s[1];
-    cell[o] = ci;
-    ++o;
+  for (int j0 = 0; j0 < len && o < end; j0 += kSnapRows) {
+    // j0 is a multiple of kSnapRows, so a new mask word starts only at u = 0
+    if ((j0 & 63) == 0) m = mask[(size_t)(j0 >> 6) * nc_pad + ci];
+    V4 a[kSnapRows], b[kSnapRows];
+    bool occ[kSnapRows];
+#pragma unroll
+    for (int u = 0; u < kSnapRows; ++u) {
+      const int j = j0 + u;
+      occ[u] = j < len && ((m >> (j & 63)) & 1ull);
+      if (occ[u]) {
+        const V4* s = reinterpret_cast<const V4*>(cols + col_slot(ci, j, D) * kSlotLen);
+        a[u] = s[0];
+        b[u] = s[1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kSnapRows; ++u) {
+      if (occ[u] && o < end) {
+        V4* d = reinterpret_cast<V4*>(slots + (size_t)o * kSlotLen);
+        d[0] = a[u];
+        d[1] = b[u];
+        cell[o] = ci;
+        ++o;
+      }
+    }
   }
 }
 
